@@ -1,0 +1,204 @@
+/*
+ * starflood_ext.h -- additive entry points of libstarflood_hip.so that have
+ * no counterpart symbol in the reference.
+ *
+ * The reference configures everything at compile time (src/config.h); here
+ * the same knobs are run-time parameters whose defaults are the config.h
+ * values.  The other entry points exist because the device is a separate
+ * memory: explicit host<->device synchronisation for the device-resident
+ * mode, device pointers for zero-copy interop (PyTorch), FP64 statistics,
+ * kernel timing, and the adjoint of the force and of the KDK step.
+ */
+#ifndef STARFLOOD_EXT_H
+#define STARFLOOD_EXT_H
+
+#include "starflood.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Arithmetic policy of the force, KDK and statistics kernels. */
+enum sf_precision {
+	/* Reference operation order in FP32: IEEE divide and sqrt, no FMA, per
+	   pair Kahan, sequential j order per particle, FP32 drift, sequential
+	   FP64 Kahan energy sums (reference src/gravity.c:693-812,
+	   src/simulation.c:381-497).  Bit-identical to the reference CPU path. */
+	SF_PRECISION_EXACT = 0,
+	/* Throughput policy: v_rsq_f32 and FMA on the FP32 VALU, per-tile
+	   partial sums combined with compensation, FP64 position drift,
+	   deterministic tree-shaped FP64 reductions. */
+	SF_PRECISION_FAST = 1
+};
+
+/* What the "acc" block holds. */
+enum sf_acc_mode {
+	/* acc_i = -G m_i sum_j m_j r_ij / (r^2+eps^2)^(3/2): the force, which the
+	   reference's kick uses as an acceleration (src/gravity.c:808-810,
+	   src/simulation.c:382).  Default. */
+	SF_ACC_REFERENCE = 0,
+	/* acc_i = -G sum_j m_j r_ij / (r^2+eps^2)^(3/2): the acceleration; the
+	   SPH term likewise omits the m_i factor of src/sph.c:274-276. */
+	SF_ACC_PHYSICAL = 1
+};
+
+/* Barnes-Hut flavour used by tree_build / solve_gravity_part_tree. */
+enum sf_tree_mode {
+	/* Reference octree: serial insertion in index order, leaf capacity 8,
+	   open iff bodies > 8 and (mean width / distance to centre)^2 >= theta,
+	   leaves used as monopoles with self-removal (src/tree.c:74-318,
+	   src/gravity.c:273-660).  Bit-identical node array, acc and pot. */
+	SF_TREE_REFERENCE = 0,
+	/* GPU build from 63-bit Morton codes (FP64 quantisation), stable radix
+	   sort, binary radix tree, fixed-order monopole aggregation; wavefront
+	   coherent traversal with a group opening test and exact direct sums
+	   over leaf particles. */
+	SF_TREE_LBVH = 1
+};
+
+/* SPH formulation used by solve_sph. */
+enum sf_sph_model {
+	/* Gaussian kernel, polytrope P = (0.1 rho)^2, r^2 > 1 cutoff
+	   (src/sph.c:14-283). */
+	SF_SPH_REFERENCE = 0,
+	/* Cubic spline (support radius h, W = 8/(pi h^3) {1-6q^2+6q^3; 2(1-q)^3}),
+	   ideal gas P = (gamma-1) rho u with evolved specific internal energy u. */
+	SF_SPH_CUBIC_IDEAL = 1
+};
+
+/* Where sim.mem is authoritative between calls. */
+enum sf_mem_mode {
+	/* Every entry point uploads the blocks it reads and downloads the blocks
+	   it writes, so host sim.mem is authoritative after each call returns,
+	   as the reference's OpenMP offload maps do (src/gravity.c:688). */
+	SF_MEM_MIRROR = 0,
+	/* The device copy is authoritative; host sim.mem changes only through
+	   sf_sync().  Entry points do no host transfers. */
+	SF_MEM_RESIDENT = 1
+};
+
+typedef struct {
+	double epsilon;              /* EPSILON, config.h:173 (1e-3) */
+	double G;                    /* G, config.h:178 (1) */
+	double dt;                   /* TIMESTEP_SIZE, config.h:170 (1) */
+	double theta;                /* TREE_THETA, config.h:44 (0.5) */
+	unsigned int tree_nodes_max; /* TREE_NODES_MAX, config.h:41 (262144) */
+	unsigned int tree_depth_max; /* TREE_DEPTH_MAX, config.h:38 (16, at most 16) */
+	unsigned int leaf_size;      /* SF_TREE_LBVH leaf capacity (16) */
+	int enable_tree;             /* ENABLE_TREE, config.h:35 (1) */
+	int enable_sph;              /* ENABLE_SPH, config.h:190 (0) */
+	int kahan_solver;            /* ENABLE_KAHAN_SUMMATION_SOLVER, config.h:194 (1) */
+	int kahan_energy;            /* ENABLE_KAHAN_SUMMATION_ENERGY, config.h:198 (1) */
+	int precision;               /* enum sf_precision (EXACT) */
+	int acc_mode;                /* enum sf_acc_mode (REFERENCE) */
+	int tree_mode;               /* enum sf_tree_mode (REFERENCE) */
+	int sph_model;               /* enum sf_sph_model (REFERENCE) */
+	int mem_mode;                /* enum sf_mem_mode (MIRROR) */
+	double sph_gamma;            /* SF_SPH_CUBIC_IDEAL adiabatic index (5/3) */
+	int logging;                 /* write <out_dir>/statistics.csv and
+	                                timings_sim_step.csv like the reference
+	                                (config.h:120-123) (1) */
+	int verbose;                 /* print energies per step (simulation.c:503) (0) */
+	char out_dir[256];           /* OUTPUT_DIR, config.h:75 ("./out") */
+} sf_params_t;
+
+/* Fill *p with the defaults listed above. */
+void sf_params_default(sf_params_t* p);
+/* Replace the process-wide parameters.  Fails if a live simulation was
+   initialised with a different enable_sph (the sim.mem layout). */
+int sf_set_params(const sf_params_t* p);
+int sf_get_params(sf_params_t* p);
+
+/* Message of the last failure in this thread ("" if none). */
+const char* sf_last_error(void);
+
+/* Library version string. */
+const char* sf_version(void);
+
+/* Number of reals per particle in sim.mem under the current parameters:
+   13, or 16 with enable_sph (reference enum sim_conf). */
+unsigned int sf_sim_dof(void);
+
+enum sf_sync_dir { SF_SYNC_TO_DEVICE = 0, SF_SYNC_TO_HOST = 1 };
+/* Copy the whole of sim.mem between host and device. */
+int sf_sync(const sim_t* sim_ptr, int direction);
+
+/* Device address of a sim.mem block (same layout as sim_find), for
+   zero-copy interop.  NULL on error. */
+void* sf_device_find(const sim_t* sim_ptr, enum sim_conf parameter_enum);
+
+/* Extension per-particle arrays that are not part of sim.mem. */
+enum sf_ext_array {
+	SF_EXT_U = 0,      /* specific internal energy (SF_SPH_CUBIC_IDEAL), N floats */
+	SF_EXT_DUDT = 1,   /* its time derivative, N floats */
+	SF_EXT_X64 = 2     /* FP64 positions used by the FAST drift, 3N doubles */
+};
+void* sf_device_ext(const sim_t* sim_ptr, int which);
+/* Host<->device copy of an extension array (count elements). */
+int sf_ext_copy(const sim_t* sim_ptr, int which, void* host, int direction);
+
+typedef struct {
+	double T;        /* sum ken (FP64), as src/simulation.c:455-475 */
+	double V;        /* sum pen (FP64), as src/simulation.c:477-497 */
+	double E;        /* T + V, as logged by the reference */
+	double W;        /* pair potential sum_{i<j} -G m_i m_j / s_ij = 0.5 sum_i m_i (pot_i + G m_i / eps) */
+	double U;        /* sum m_i u_i (SF_SPH_CUBIC_IDEAL), else 0 */
+	double P[3];     /* linear momentum sum m v (FP64) */
+	double L[3];     /* angular momentum sum m x cross v (FP64) */
+} sf_stats_t;
+/* Statistics of the current device state (the T, V of the last sim_step
+   are the values it logged). */
+int sf_get_stats(const sim_t* sim_ptr, sf_stats_t* out);
+
+/* --- initial conditions (synthetic, seeded, FP64 on the host) ---
+   Each generator writes host sim.mem (pos, vel, mas; rad is left to the
+   caller) and, in SF_MEM_RESIDENT mode, then uploads it. */
+/* Uniform ball of the given radius centred on the origin: r = radius *
+   cbrt(u), isotropic direction (as src/initcond.c:73-86), velocity
+   v = v_hubble * x, masses total_mass / N. */
+int sf_ic_uniform_ball(sim_t* sim_ptr, uint32_t seed, double radius, double total_mass, double v_hubble);
+/* Plummer sphere (scale a, mass M) on particles [first, first+count):
+   radii by inverse CDF, velocities by Aarseth-Henon-Wielen rejection, then
+   shifted by centre and bulk velocity.  Masses M / count. */
+int sf_ic_plummer(sim_t* sim_ptr, uint32_t seed, unsigned int first, unsigned int count,
+                  double a, double M, const double centre[3], const double bulk_v[3]);
+
+/* --- kernel timing with HIP events on the library stream --- */
+void sf_prof_enable(int on);
+void sf_prof_reset(void);
+/* Total milliseconds and launch count of kernels recorded under `name`
+   ("pp_fast", "pp_exact", "kdk", ...).  Synchronises the library stream. */
+int sf_prof_query(const char* name, double* total_ms, unsigned long* count);
+
+/* FP64 direct sum on the GPU for the listed particles (host index list, host
+   outputs 3*n_idx and n_idx doubles), same acc_mode semantics as the
+   solvers, self pair included.  A precision reference, not a solver. */
+int sf_pp_direct_f64(const sim_t* sim_ptr, const unsigned int* idx, unsigned int n_idx,
+                     double* acc_out, double* pot_out);
+
+/* --- adjoint of the force and of the KDK step (device pointers) ---
+   State: x, v are 3N floats (AoS), m is N floats, all device memory.
+   The tape keeps (x, v) every `ckpt_every` steps on the device. */
+typedef struct sf_tape sf_tape;
+/* Run `nsteps` KDK steps of PP gravity from (x0, v0) with the current
+   epsilon, G, dt and acc_mode; write the final state to (xN, vN) and return
+   a tape for sf_adjoint_backward.  `stream` is a hipStream_t (NULL: the
+   library stream). */
+int sf_adjoint_forward(unsigned int N, const float* x0, const float* v0, const float* m,
+                       unsigned int nsteps, unsigned int ckpt_every,
+                       float* xN, float* vN, void* stream, sf_tape** tape_out);
+/* Given dL/dxN and dL/dvN (device, 3N each; either may be NULL for zero),
+   write dL/dx0 and dL/dv0 (device, 3N each). */
+int sf_adjoint_backward(sf_tape* tape, const float* gxN, const float* gvN,
+                        float* gx0, float* gv0, void* stream);
+void sf_tape_free(sf_tape* tape);
+/* One force evaluation on device arrays: acc (3N) from x (3N) and m (N). */
+int sf_force_dev(unsigned int N, const float* x, const float* m, float* acc, float* pot, void* stream);
+/* Force vector-Jacobian product: g = (d acc / d x)^T mu, device arrays. */
+int sf_force_vjp_dev(unsigned int N, const float* x, const float* m, const float* mu, float* g, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STARFLOOD_EXT_H */

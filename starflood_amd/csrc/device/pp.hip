@@ -1,0 +1,335 @@
+// pp.hip -- particle-particle gravity on gfx950.
+//
+// Two kernels compute the same quantity as the reference's
+// solve_gravity_part_part (src/gravity.c:666-819):
+//   sum_i = sum_j m_j r_ij / (r_ij^2 + eps^2)^(3/2),  r_ij = r_i - r_j
+//   acc_i = -G m_i sum_i   (SF_ACC_REFERENCE)   or  -G sum_i (SF_ACC_PHYSICAL)
+//   pot_i = -G sum_j m_j / sqrt(r_ij^2 + eps^2)
+// with the self pair included (its force term is zero, its potential term is
+// -G m_i / eps), as the reference does.
+//
+// k_pp_exact keeps the reference's FP32 operation order (no contraction: the
+// file is compiled with -ffp-contract=off; IEEE divide and square root;
+// per-pair Kahan; j in index order), so its output is bit-identical to the
+// CPU path.  One lane owns one i; the j stream is wavefront-uniform, so the
+// compiler fetches it with scalar loads into SGPRs.
+//
+// k_pp_fast is the throughput kernel: 13 VALU operations plus one v_rsq_f32
+// per interaction, several i per lane for instruction-level parallelism, j
+// fetched with scalar loads, partial sums per tile of SF_TJ interactions
+// folded into a Kahan-compensated running sum, and the j range split into
+// slices so the grid fills all 256 CUs.  Slice s is processed by blocks with
+// blockIdx % slices == s, which the dispatcher places on XCD (s mod 8), so
+// each XCD's L2 holds only its own slice of the body array.  A second kernel
+// combines the slice partials per i in fixed order in FP64, which keeps the
+// result deterministic.
+#include <hip/hip_runtime.h>
+
+#include "../sf_internal.h"
+
+namespace {
+
+constexpr unsigned kBlock = 256;
+constexpr unsigned SF_TJ = 128;   // interactions per plain-FP32 partial sum
+
+// One step of Kahan summation (reference src/gravity.c:771-784):
+// y = x - c; t = s + y; c = (t - s) - y; s = t.
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
+	const float y = x - c;
+	const float t = s + y;
+	const float z = t - s;
+	c = z - y;
+	s = t;
+}
+
+template <bool KAHAN_ACC, bool KAHAN_POT, bool PHYS>
+__global__ __launch_bounds__(kBlock) void k_pp_exact(const float* __restrict__ pos, const float* __restrict__ mas,
+                                                     unsigned N, float eps2, float negG,
+                                                     float* __restrict__ acc, float* __restrict__ pot) {
+	const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+	const bool live = i < N;
+	const unsigned ii = live ? i : N - 1u;
+	const float xi = pos[3u * ii + 0u];
+	const float yi = pos[3u * ii + 1u];
+	const float zi = pos[3u * ii + 2u];
+	const float mi = mas[ii];
+
+	float sx = 0.0f, sy = 0.0f, sz = 0.0f, sp = 0.0f;
+	float cx = 0.0f, cy = 0.0f, cz = 0.0f, cp = 0.0f;
+
+	for (unsigned j = 0; j < N; ++j) {
+		const float mj = mas[j];
+		const float rx = xi - pos[3u * j + 0u];
+		const float ry = yi - pos[3u * j + 1u];
+		const float rz = zi - pos[3u * j + 2u];
+		const float r2 = (rx * rx + ry * ry) + rz * rz;
+		const float inv_r2 = 1.0f / (r2 + eps2);
+		const float inv_r1 = 1.0f / sqrtf(r2 + eps2);
+		const float pj = mj * inv_r1;
+		const float fx = pj * inv_r2 * rx;
+		const float fy = pj * inv_r2 * ry;
+		const float fz = pj * inv_r2 * rz;
+		if (KAHAN_ACC) {
+			kahan_add(sx, cx, fx);
+			kahan_add(sy, cy, fy);
+			kahan_add(sz, cz, fz);
+		} else {
+			sx += fx;
+			sy += fy;
+			sz += fz;
+		}
+		if (KAHAN_POT) {
+			kahan_add(sp, cp, pj);
+		} else {
+			sp += pj;
+		}
+	}
+	if (live) {
+		// reference src/gravity.c:808-811: ((-G) * m_i) * acc_sum, (-G) * pot_sum
+		const float f = PHYS ? negG : negG * mi;
+		acc[3u * i + 0u] = f * sx;
+		acc[3u * i + 1u] = f * sy;
+		acc[3u * i + 2u] = f * sz;
+		pot[i] = negG * sp;
+	}
+}
+
+// Accumulates, for IPT particles per lane, sum_j m_j (r_j - r_i) / s^3 and
+// sum_j m_j / s over one j slice.  part holds, per slice, two planes of
+// n_pad float4: the running sum and its Kahan compensation.
+template <int IPT, bool WITH_POT>
+__global__ __launch_bounds__(kBlock) void k_pp_fast(const float4* __restrict__ body, unsigned n_pad,
+                                                    unsigned slices, unsigned slice_len, float eps2,
+                                                    float4* __restrict__ part) {
+	const unsigned b = blockIdx.x;
+	const unsigned s = b % slices;
+	const unsigned ib = b / slices;
+	const unsigned ibase = ib * (kBlock * IPT) + threadIdx.x;
+
+	float xi[IPT], yi[IPT], zi[IPT];
+	float Sx[IPT], Sy[IPT], Sz[IPT], Sp[IPT];
+	float Cx[IPT], Cy[IPT], Cz[IPT], Cp[IPT];
+#pragma unroll
+	for (int k = 0; k < IPT; ++k) {
+		const float4 bi = body[ibase + k * kBlock];
+		xi[k] = bi.x;
+		yi[k] = bi.y;
+		zi[k] = bi.z;
+		Sx[k] = Sy[k] = Sz[k] = Sp[k] = 0.0f;
+		Cx[k] = Cy[k] = Cz[k] = Cp[k] = 0.0f;
+	}
+
+	const unsigned j0 = s * slice_len;
+	const unsigned j1 = min(j0 + slice_len, n_pad);
+	for (unsigned jt = j0; jt < j1; jt += SF_TJ) {
+		float tx[IPT], ty[IPT], tz[IPT], tp[IPT];
+#pragma unroll
+		for (int k = 0; k < IPT; ++k) tx[k] = ty[k] = tz[k] = tp[k] = 0.0f;
+#pragma unroll 4
+		for (unsigned j = jt; j < jt + SF_TJ; ++j) {
+			const float4 bj = body[j];
+#pragma unroll
+			for (int k = 0; k < IPT; ++k) {
+				const float dx = bj.x - xi[k];
+				const float dy = bj.y - yi[k];
+				const float dz = bj.z - zi[k];
+				float r2 = __builtin_fmaf(dx, dx, eps2);
+				r2 = __builtin_fmaf(dy, dy, r2);
+				r2 = __builtin_fmaf(dz, dz, r2);
+				const float inv = __builtin_amdgcn_rsqf(r2);
+				const float mi = bj.w * inv;
+				const float inv2 = inv * inv;
+				const float w = mi * inv2;
+				tx[k] = __builtin_fmaf(w, dx, tx[k]);
+				ty[k] = __builtin_fmaf(w, dy, ty[k]);
+				tz[k] = __builtin_fmaf(w, dz, tz[k]);
+				if (WITH_POT) tp[k] += mi;
+			}
+		}
+#pragma unroll
+		for (int k = 0; k < IPT; ++k) {
+			kahan_add(Sx[k], Cx[k], tx[k]);
+			kahan_add(Sy[k], Cy[k], ty[k]);
+			kahan_add(Sz[k], Cz[k], tz[k]);
+			if (WITH_POT) kahan_add(Sp[k], Cp[k], tp[k]);
+		}
+	}
+
+	float4* sum_plane = part + (size_t)(2u * s) * n_pad;
+	float4* com_plane = part + (size_t)(2u * s + 1u) * n_pad;
+#pragma unroll
+	for (int k = 0; k < IPT; ++k) {
+		const unsigned i = ibase + k * kBlock;
+		sum_plane[i] = make_float4(Sx[k], Sy[k], Sz[k], Sp[k]);
+		com_plane[i] = make_float4(Cx[k], Cy[k], Cz[k], Cp[k]);
+	}
+}
+
+// Fixed-order FP64 combine of the slice partials; Kahan's compensation c
+// holds the negated lost low part, so each slice contributes (s - c).
+__global__ __launch_bounds__(kBlock) void k_pp_combine(const float4* __restrict__ part, unsigned n_pad,
+                                                       unsigned slices, const float4* __restrict__ body,
+                                                       unsigned N, double G, int acc_phys,
+                                                       float* __restrict__ acc, float* __restrict__ pot) {
+	const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= N) return;
+	double sx = 0.0, sy = 0.0, sz = 0.0, sp = 0.0;
+	for (unsigned s = 0; s < slices; ++s) {
+		const float4 a = part[(size_t)(2u * s) * n_pad + i];
+		const float4 c = part[(size_t)(2u * s + 1u) * n_pad + i];
+		sx += (double)a.x - (double)c.x;
+		sy += (double)a.y - (double)c.y;
+		sz += (double)a.z - (double)c.z;
+		sp += (double)a.w - (double)c.w;
+	}
+	// sum_j m_j (r_j - r_i)/s^3 = -sum_j m_j r_ij/s^3, so acc = G (m_i) * sum
+	const double f = acc_phys ? G : G * (double)body[i].w;
+	acc[3u * i + 0u] = (float)(f * sx);
+	acc[3u * i + 1u] = (float)(f * sy);
+	acc[3u * i + 2u] = (float)(f * sz);
+	if (pot) pot[i] = (float)(-G * sp);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pack_bodies(const float* __restrict__ pos, const float* __restrict__ mas,
+                                                        unsigned N, unsigned n_pad, float4* __restrict__ body) {
+	const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n_pad) return;
+	body[i] = i < N ? make_float4(pos[3u * i], pos[3u * i + 1u], pos[3u * i + 2u], mas[i])
+	                : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// FP64 direct sum: one block per listed particle, lanes stride over j, then
+// a fixed-shape tree reduction in LDS.
+__global__ __launch_bounds__(kBlock) void k_pp_f64(const float* __restrict__ pos, const float* __restrict__ mas,
+                                                   unsigned N, double eps2, double G, int acc_phys,
+                                                   const unsigned* __restrict__ idx,
+                                                   double* __restrict__ acc, double* __restrict__ pot) {
+	__shared__ double red[4][kBlock];
+	const unsigned i = idx[blockIdx.x];
+	const double xi = pos[3u * i], yi = pos[3u * i + 1u], zi = pos[3u * i + 2u];
+	double sx = 0.0, sy = 0.0, sz = 0.0, sp = 0.0;
+	for (unsigned j = threadIdx.x; j < N; j += kBlock) {
+		const double rx = xi - (double)pos[3u * j];
+		const double ry = yi - (double)pos[3u * j + 1u];
+		const double rz = zi - (double)pos[3u * j + 2u];
+		const double s2 = rx * rx + ry * ry + rz * rz + eps2;
+		const double inv1 = 1.0 / sqrt(s2);
+		const double pj = (double)mas[j] * inv1;
+		const double w = pj * inv1 * inv1;
+		sx += w * rx;
+		sy += w * ry;
+		sz += w * rz;
+		sp += pj;
+	}
+	red[0][threadIdx.x] = sx;
+	red[1][threadIdx.x] = sy;
+	red[2][threadIdx.x] = sz;
+	red[3][threadIdx.x] = sp;
+	__syncthreads();
+	for (unsigned h = kBlock / 2; h > 0; h >>= 1) {
+		if (threadIdx.x < h) {
+#pragma unroll
+			for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + h];
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		const double f = acc_phys ? -G : -G * (double)mas[i];
+		acc[3u * blockIdx.x + 0u] = f * red[0][0];
+		acc[3u * blockIdx.x + 1u] = f * red[1][0];
+		acc[3u * blockIdx.x + 2u] = f * red[2][0];
+		pot[blockIdx.x] = -G * red[3][0];
+	}
+}
+
+int env_uint(const char* name, int fallback) {
+	const char* v = getenv(name);
+	if (!v || !*v) return fallback;
+	return atoi(v);
+}
+
+}  // namespace
+
+extern "C" int sfd_pack_bodies(const float* pos, const float* mas, unsigned N, unsigned n_pad, float* body,
+                               hipStream_t s) {
+	const unsigned grid = (n_pad + kBlock - 1) / kBlock;
+	hipLaunchKernelGGL(k_pack_bodies, dim3(grid), dim3(kBlock), 0, s, pos, mas, N, n_pad,
+	                   reinterpret_cast<float4*>(body));
+	return sf_hip_check(hipGetLastError(), "sfd_pack_bodies", "k_pack_bodies launch");
+}
+
+extern "C" int sfd_pp_exact(const float* pos, const float* mas, unsigned N, float eps2, float negG, int acc_phys,
+                            int kahan_acc, int kahan_pot, float* acc, float* pot, hipStream_t s) {
+	if (N == 0) return 0;
+	const dim3 grid((N + kBlock - 1) / kBlock), block(kBlock);
+	const int sel = (kahan_acc ? 4 : 0) | (kahan_pot ? 2 : 0) | (acc_phys ? 1 : 0);
+	const int tok = sf_prof_begin("pp_exact", s);
+	switch (sel) {
+	case 0: hipLaunchKernelGGL((k_pp_exact<false, false, false>), grid, block, 0, s, pos, mas, N, eps2, negG, acc, pot); break;
+	case 1: hipLaunchKernelGGL((k_pp_exact<false, false, true>), grid, block, 0, s, pos, mas, N, eps2, negG, acc, pot); break;
+	case 2: hipLaunchKernelGGL((k_pp_exact<false, true, false>), grid, block, 0, s, pos, mas, N, eps2, negG, acc, pot); break;
+	case 3: hipLaunchKernelGGL((k_pp_exact<false, true, true>), grid, block, 0, s, pos, mas, N, eps2, negG, acc, pot); break;
+	case 4: hipLaunchKernelGGL((k_pp_exact<true, false, false>), grid, block, 0, s, pos, mas, N, eps2, negG, acc, pot); break;
+	case 5: hipLaunchKernelGGL((k_pp_exact<true, false, true>), grid, block, 0, s, pos, mas, N, eps2, negG, acc, pot); break;
+	case 6: hipLaunchKernelGGL((k_pp_exact<true, true, false>), grid, block, 0, s, pos, mas, N, eps2, negG, acc, pot); break;
+	default: hipLaunchKernelGGL((k_pp_exact<true, true, true>), grid, block, 0, s, pos, mas, N, eps2, negG, acc, pot); break;
+	}
+	sf_prof_end(tok, s);
+	return sf_hip_check(hipGetLastError(), "sfd_pp_exact", "k_pp_exact launch");
+}
+
+extern "C" int sfd_pp_fast(const float* body, unsigned N, unsigned n_pad, float eps2, double G, int acc_phys,
+                           float* part, unsigned slices, float* acc, float* pot, hipStream_t s) {
+	if (N == 0) return 0;
+	static int ipt = -1;
+	if (ipt < 0) {
+		ipt = env_uint("SF_PP_IPT", 1);
+		if (ipt != 1 && ipt != 2 && ipt != 4 && ipt != 8) ipt = 1;
+	}
+	const unsigned per_block = kBlock * (unsigned)ipt;
+	if (n_pad % per_block != 0 || n_pad % SF_TJ != 0) {
+		sf_error("sfd_pp_fast", "n_pad %u is not a multiple of %u", n_pad, per_block);
+		return -1;
+	}
+	// slice length: whole SF_TJ tiles, every slice non-empty
+	unsigned tiles = n_pad / SF_TJ;
+	if (slices > tiles) slices = tiles;
+	const unsigned tiles_per_slice = (tiles + slices - 1) / slices;
+	const unsigned slice_len = tiles_per_slice * SF_TJ;
+	slices = (n_pad + slice_len - 1) / slice_len;
+	const unsigned grid = (n_pad / per_block) * slices;
+	float4* p4 = reinterpret_cast<float4*>(part);
+	const float4* b4 = reinterpret_cast<const float4*>(body);
+	const int tok = sf_prof_begin("pp_fast", s);
+	if (pot) {
+		switch (ipt) {
+		case 1: hipLaunchKernelGGL((k_pp_fast<1, true>), dim3(grid), dim3(kBlock), 0, s, b4, n_pad, slices, slice_len, eps2, p4); break;
+		case 2: hipLaunchKernelGGL((k_pp_fast<2, true>), dim3(grid), dim3(kBlock), 0, s, b4, n_pad, slices, slice_len, eps2, p4); break;
+		case 8: hipLaunchKernelGGL((k_pp_fast<8, true>), dim3(grid), dim3(kBlock), 0, s, b4, n_pad, slices, slice_len, eps2, p4); break;
+		default: hipLaunchKernelGGL((k_pp_fast<4, true>), dim3(grid), dim3(kBlock), 0, s, b4, n_pad, slices, slice_len, eps2, p4); break;
+		}
+	} else {
+		switch (ipt) {
+		case 1: hipLaunchKernelGGL((k_pp_fast<1, false>), dim3(grid), dim3(kBlock), 0, s, b4, n_pad, slices, slice_len, eps2, p4); break;
+		case 2: hipLaunchKernelGGL((k_pp_fast<2, false>), dim3(grid), dim3(kBlock), 0, s, b4, n_pad, slices, slice_len, eps2, p4); break;
+		case 8: hipLaunchKernelGGL((k_pp_fast<8, false>), dim3(grid), dim3(kBlock), 0, s, b4, n_pad, slices, slice_len, eps2, p4); break;
+		default: hipLaunchKernelGGL((k_pp_fast<4, false>), dim3(grid), dim3(kBlock), 0, s, b4, n_pad, slices, slice_len, eps2, p4); break;
+		}
+	}
+	sf_prof_end(tok, s);
+	if (sf_hip_check(hipGetLastError(), "sfd_pp_fast", "k_pp_fast launch")) return -1;
+	const int tok2 = sf_prof_begin("pp_combine", s);
+	hipLaunchKernelGGL(k_pp_combine, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p4, n_pad, slices, b4, N,
+	                   G, acc_phys, acc, pot);
+	sf_prof_end(tok2, s);
+	return sf_hip_check(hipGetLastError(), "sfd_pp_fast", "k_pp_combine launch");
+}
+
+extern "C" int sfd_pp_f64(const float* pos, const float* mas, unsigned N, double eps2, double G, int acc_phys,
+                          const unsigned* idx, unsigned n_idx, double* acc, double* pot, hipStream_t s) {
+	if (n_idx == 0) return 0;
+	const int tok = sf_prof_begin("pp_f64", s);
+	hipLaunchKernelGGL(k_pp_f64, dim3(n_idx), dim3(kBlock), 0, s, pos, mas, N, eps2, G, acc_phys, idx, acc, pot);
+	sf_prof_end(tok, s);
+	return sf_hip_check(hipGetLastError(), "sfd_pp_f64", "k_pp_f64 launch");
+}

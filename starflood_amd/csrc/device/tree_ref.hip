@@ -1,0 +1,302 @@
+// tree_ref.hip -- the reference's octree (src/tree.c:74-318) and its walk
+// (src/gravity.c:245-663), reproduced bit for bit on the device.
+//
+// The reference inserts particles one at a time in index order; node ids,
+// the per-node FP32 mass-moment sums and the leaf particle order all depend
+// on that order, so the build is one lane walking the particles in order
+// (SF_TREE_LBVH is the parallel build).  The walk is one lane per particle
+// with the reference's parent-pointer walk, opening test and leaf-monopole
+// self-removal, in the reference's FP32 operation order (the file is
+// compiled with -ffp-contract=off).
+#include <hip/hip_runtime.h>
+
+#include "../sf_internal.h"
+#include "tree_dev.h"
+
+namespace {
+
+constexpr unsigned kBlock = 256;
+constexpr int kDepthCap = 16;   // TREE_DEPTH_MAX upper limit (tree_path size)
+
+// Mean radius (sequential FP64 Kahan, src/simulation.c:245-277) and the
+// jittered cube bounds (src/simulation.c:279-301).  jitter[i] is
+// 2 * (INV_PCG32_MAX * s[i]) - 1 from pcg4d of the step number.
+__global__ void k_tree_bounds_ref(const float* __restrict__ pos, unsigned N, double j0, double j1, double j2,
+                                  float* __restrict__ bounds) {
+	if (blockIdx.x != 0 || threadIdx.x != 0) return;
+	double avg = 0.0, com = 0.0;
+	for (unsigned i = 0; i < N; ++i) {
+		const double x = pos[3u * i], y = pos[3u * i + 1u], z = pos[3u * i + 2u];
+		const double r1 = sqrt((x * x + y * y) + z * z);
+		const double yk = r1 - com;
+		const double t = avg + yk;
+		const double zk = t - avg;
+		com = zk - yk;
+		avg = t;
+	}
+	avg *= 1.0 / (double)N;
+	const double jit[3] = {j0, j1, j2};
+	for (int i = 0; i < 3; ++i) {
+		bounds[i] = (float)(-2.000 * avg + 0.5 * jit[i] * avg);
+		bounds[3 + i] = (float)(2.000 * avg + 0.5 * jit[i] * avg);
+	}
+}
+
+// Serial insertion (src/tree.c:120-296).  meta[0] = num_nodes,
+// meta[1] = status (0 ok, 1 out of nodes, 2 too deep).
+__global__ void k_tree_build_ref(const float* __restrict__ pos, const float* __restrict__ mas, unsigned N,
+                                 const float* __restrict__ tbounds, unsigned max_nodes, int depth_max,
+                                 sf_node* __restrict__ node, unsigned* __restrict__ meta) {
+	if (blockIdx.x != 0 || threadIdx.x != 0) return;
+	unsigned num_nodes = 1u;
+	float tb[6];
+	for (int k = 0; k < 6; ++k) tb[k] = tbounds[k];
+	// purge of the root (nodes past num_nodes are never read)
+	{
+		sf_node& r = node[0];
+		r.parent = 0u;
+		r.bodies = 0u;
+		for (int k = 0; k < 8; ++k) r.branch[k] = 0u;
+		for (int k = 0; k < 6; ++k) r.bounds[k] = tb[k];
+		for (int k = 0; k < 4; ++k) r.params[k] = 0.0f;
+	}
+	unsigned status = 0u;
+	for (unsigned idx = 0; idx < N && status == 0u; ++idx) {
+		const float px = pos[3u * idx], py = pos[3u * idx + 1u], pz = pos[3u * idx + 2u];
+		const float pm = mas[idx];
+		float bmin[3] = {tb[0], tb[1], tb[2]};
+		float bmax[3] = {tb[3], tb[4], tb[5]};
+		const float p[3] = {px, py, pz};
+		if (bmin[0] > px || px > bmax[0] || bmin[1] > py || py > bmax[1] || bmin[2] > pz || pz > bmax[2]) continue;
+		unsigned cur = 0u;
+		for (int depth = 0; depth <= depth_max; ++depth) {
+			sf_node& cn = node[cur];
+			const unsigned cur_bodies = cn.bodies;
+			cn.bodies = cur_bodies + 1u;
+			float mid[3];
+			for (int a = 0; a < 3; ++a) mid[a] = 0.5f * (bmin[a] + bmax[a]);
+			cn.params[0] = cn.params[0] + pm * px;
+			cn.params[1] = cn.params[1] + pm * py;
+			cn.params[2] = cn.params[2] + pm * pz;
+			cn.params[3] = cn.params[3] + pm;
+			if (cur_bodies < 8u) {
+				cn.branch[cur_bodies] = idx;
+				break;
+			}
+			if (cur_bodies == 8u) {
+				if (max_nodes <= num_nodes + 8u) {
+					status = 1u;
+					break;
+				}
+				if (depth_max <= depth) {
+					status = 2u;
+					break;
+				}
+				for (int o = 0; o < 8; ++o) {
+					sf_node& nn = node[num_nodes + (unsigned)o];
+					nn.parent = cur;
+					nn.bodies = 0u;
+					for (int k = 0; k < 8; ++k) nn.branch[k] = 0u;
+					for (int a = 0; a < 3; ++a) {
+						nn.bounds[a] = ((o >> a) & 1) ? mid[a] : bmin[a];
+						nn.bounds[3 + a] = ((o >> a) & 1) ? bmax[a] : mid[a];
+					}
+					for (int k = 0; k < 4; ++k) nn.params[k] = 0.0f;
+				}
+				// redistribution uses '<' (src/tree.c:244-247)
+				for (int k = 0; k < 8; ++k) {
+					const unsigned ci = cn.branch[k];
+					const float cx = pos[3u * ci], cy = pos[3u * ci + 1u], cz = pos[3u * ci + 2u];
+					const float cm = mas[ci];
+					const int oc = (mid[0] < cx ? 1 : 0) + (mid[1] < cy ? 2 : 0) + (mid[2] < cz ? 4 : 0);
+					sf_node& ch = node[num_nodes + (unsigned)oc];
+					const unsigned cb = ch.bodies;
+					ch.bodies = cb + 1u;
+					ch.branch[cb] = ci;
+					ch.params[0] = ch.params[0] + cm * cx;
+					ch.params[1] = ch.params[1] + cm * cy;
+					ch.params[2] = ch.params[2] + cm * cz;
+					ch.params[3] = ch.params[3] + cm;
+				}
+				for (int o = 0; o < 8; ++o) cn.branch[o] = num_nodes + (unsigned)o;
+				num_nodes += 8u;
+			}
+			// insertion uses '<=' (src/tree.c:282-285)
+			const int oct = (mid[0] <= p[0] ? 1 : 0) + (mid[1] <= p[1] ? 2 : 0) + (mid[2] <= p[2] ? 4 : 0);
+			for (int a = 0; a < 3; ++a) {
+				const bool hi = (oct >> a) & 1;
+				bmin[a] = hi ? mid[a] : bmin[a];
+				bmax[a] = hi ? bmax[a] : mid[a];
+			}
+			cur = cn.branch[oct];
+		}
+	}
+	meta[0] = num_nodes;
+	meta[1] = status;
+}
+
+// Centre of mass: moment sums divided by the mass (src/tree.c:302-310);
+// skipped after a failed build, as the reference returns before it.
+__global__ __launch_bounds__(kBlock) void k_tree_com_ref(sf_node* __restrict__ node, const unsigned* __restrict__ meta) {
+	const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+	if (meta[1] != 0u || i >= meta[0]) return;
+	const float mass = node[i].params[3];
+	const float inv = 0.0f < mass ? 1.0f / mass : 1.0f;
+	node[i].params[0] = node[i].params[0] * inv;
+	node[i].params[1] = node[i].params[1] * inv;
+	node[i].params[2] = node[i].params[2] * inv;
+}
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
+	const float y = x - c;
+	const float t = s + y;
+	const float z = t - s;
+	c = z - y;
+	s = t;
+}
+
+// Per-particle walk of src/gravity.c:273-660.
+template <bool KAHAN_ACC, bool KAHAN_POT, bool PHYS>
+__global__ __launch_bounds__(kBlock) void k_tree_walk_ref(const float* __restrict__ pos, const float* __restrict__ mas,
+                                                          unsigned N, const sf_node* __restrict__ node,
+                                                          unsigned node_cap, unsigned iter_max, float eps2,
+                                                          float theta, float negG,
+                                                          float* __restrict__ acc, float* __restrict__ pot) {
+	const unsigned idx = blockIdx.x * kBlock + threadIdx.x;
+	if (idx >= N) return;
+	float sx = 0.0f, sy = 0.0f, sz = 0.0f, sp = 0.0f;
+	float cx = 0.0f, cy = 0.0f, cz = 0.0f, cp = 0.0f;
+	const float ri[3] = {pos[3u * idx], pos[3u * idx + 1u], pos[3u * idx + 2u]};
+	const float mi = mas[idx];
+	unsigned next_node = 0u;
+	int cur_depth = 0;
+	int path[kDepthCap];
+	for (int k = 0; k < kDepthCap; ++k) path[k] = -1;
+	const float third = (float)(1.0 / 3.0);
+
+	for (unsigned it = 0; it < iter_max; ++it) {
+		if (cur_depth < 0) break;
+		// Only a failed build (a node counted past 8 bodies but never split)
+		// can lead outside the allocated nodes or below the path stack; the
+		// reference reads out of bounds there, this walk stops.
+		if (next_node >= node_cap) break;
+		const sf_node* nd = &node[next_node];
+		const unsigned bodies = nd->bodies;
+		const unsigned parent = nd->parent;
+		const float bmin[3] = {nd->bounds[0], nd->bounds[1], nd->bounds[2]};
+		const float bmax[3] = {nd->bounds[3], nd->bounds[4], nd->bounds[5]};
+		if (bodies == 0u) {
+			next_node = parent;
+			cur_depth -= 1;
+			continue;
+		}
+		if (bodies > 8u) {
+			if (cur_depth >= kDepthCap) break;   // only after a failed build
+			const int octant = path[cur_depth];
+			if (octant >= 7) {
+				next_node = parent;
+				path[cur_depth] = -1;
+				cur_depth -= 1;
+				continue;
+			}
+			if (octant < 0) {
+				float off[3];
+				for (int a = 0; a < 3; ++a) off[a] = 0.5f * (bmin[a] + bmax[a]) - ri[a];
+				const float d2 = (off[0] * off[0] + off[1] * off[1]) + off[2] * off[2];
+				const float w = third * (((bmax[0] - bmin[0]) + (bmax[1] - bmin[1])) + (bmax[2] - bmin[2]));
+				const float ratio = 0.0f < d2 ? (w * w) / d2 : 0.0f;
+				if (theta <= ratio) {
+					next_node = nd->branch[0];
+					path[cur_depth] = 0;
+					cur_depth += 1;
+					continue;
+				}
+			} else {
+				next_node = nd->branch[octant + 1];
+				path[cur_depth] = octant + 1;
+				cur_depth += 1;
+				continue;
+			}
+		}
+		// the node as one monopole
+		float rj[3] = {nd->params[0], nd->params[1], nd->params[2]};
+		float mj = nd->params[3];
+		if (bmin[0] <= ri[0] && ri[0] <= bmax[0] && bmin[1] <= ri[1] && ri[1] <= bmax[1] && bmin[2] <= ri[2] &&
+		    ri[2] <= bmax[2] && 1u < bodies) {
+			for (int a = 0; a < 3; ++a) rj[a] = rj[a] * mj;
+			for (int a = 0; a < 3; ++a) rj[a] = rj[a] - mi * ri[a];
+			mj = mj - mi;
+			for (int a = 0; a < 3; ++a) rj[a] = rj[a] * (0.0f < mj ? 1.0f / mj : 1.0f);
+		}
+		const float rx = ri[0] - rj[0], ry = ri[1] - rj[1], rz = ri[2] - rj[2];
+		const float r2 = (rx * rx + ry * ry) + rz * rz;
+		const float inv_r2 = 1.0f / (r2 + eps2);
+		const float inv_r1 = 1.0f / sqrtf(r2 + eps2);
+		const float pj = mj * inv_r1;
+		const float fx = pj * inv_r2 * rx;
+		const float fy = pj * inv_r2 * ry;
+		const float fz = pj * inv_r2 * rz;
+		if (KAHAN_ACC) {
+			kahan_add(sx, cx, fx);
+			kahan_add(sy, cy, fy);
+			kahan_add(sz, cz, fz);
+		} else {
+			sx += fx;
+			sy += fy;
+			sz += fz;
+		}
+		if (KAHAN_POT) kahan_add(sp, cp, pj);
+		else sp += pj;
+		next_node = parent;
+		cur_depth -= 1;
+	}
+	const float f = PHYS ? negG : negG * mi;
+	acc[3u * idx + 0u] = f * sx;
+	acc[3u * idx + 1u] = f * sy;
+	acc[3u * idx + 2u] = f * sz;
+	pot[idx] = negG * sp;
+}
+
+}  // namespace
+
+extern "C" int sfd_tree_bounds_ref(const float* pos, unsigned N, const double jitter[3], float* bounds, hipStream_t s) {
+	hipLaunchKernelGGL(k_tree_bounds_ref, dim3(1), dim3(64), 0, s, pos, N, jitter[0], jitter[1], jitter[2], bounds);
+	return sf_hip_check(hipGetLastError(), "sfd_tree_bounds_ref", "launch");
+}
+
+extern "C" int sfd_tree_build_ref(const float* pos, const float* mas, unsigned N, const float* bounds,
+                                  unsigned max_nodes, int depth_max, sf_node* node, unsigned* meta, hipStream_t s) {
+	const int tok = sf_prof_begin("tree_build_ref", s);
+	hipLaunchKernelGGL(k_tree_build_ref, dim3(1), dim3(64), 0, s, pos, mas, N, bounds, max_nodes, depth_max, node,
+	                   meta);
+	if (sf_hip_check(hipGetLastError(), "sfd_tree_build_ref", "build launch")) return -1;
+	const unsigned grid = (max_nodes + kBlock - 1) / kBlock;
+	hipLaunchKernelGGL(k_tree_com_ref, dim3(grid ? grid : 1), dim3(kBlock), 0, s, node, meta);
+	sf_prof_end(tok, s);
+	return sf_hip_check(hipGetLastError(), "sfd_tree_build_ref", "com launch");
+}
+
+extern "C" int sfd_tree_walk_ref(const float* pos, const float* mas, unsigned N, const sf_node* node,
+                                 unsigned node_cap, unsigned iter_max, float eps2, float theta, float negG, int acc_phys, int kahan_acc,
+                                 int kahan_pot, float* acc, float* pot, hipStream_t s) {
+	if (N == 0) return 0;
+	const dim3 grid((N + kBlock - 1) / kBlock), block(kBlock);
+	const int sel = (kahan_acc ? 4 : 0) | (kahan_pot ? 2 : 0) | (acc_phys ? 1 : 0);
+	const int tok = sf_prof_begin("tree_walk_ref", s);
+#define SF_WALK(A, B, C)                                                                                         \
+	hipLaunchKernelGGL((k_tree_walk_ref<A, B, C>), grid, block, 0, s, pos, mas, N, node, node_cap, iter_max, eps2, theta, \
+	                   negG, acc, pot)
+	switch (sel) {
+	case 0: SF_WALK(false, false, false); break;
+	case 1: SF_WALK(false, false, true); break;
+	case 2: SF_WALK(false, true, false); break;
+	case 3: SF_WALK(false, true, true); break;
+	case 4: SF_WALK(true, false, false); break;
+	case 5: SF_WALK(true, false, true); break;
+	case 6: SF_WALK(true, true, false); break;
+	default: SF_WALK(true, true, true); break;
+	}
+#undef SF_WALK
+	sf_prof_end(tok, s);
+	return sf_hip_check(hipGetLastError(), "sfd_tree_walk_ref", "walk launch");
+}

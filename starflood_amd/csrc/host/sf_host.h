@@ -1,0 +1,32 @@
+/*
+ * sf_host.h -- solver entry points shared by the host translation units.
+ * Each works on device state only (no host transfers, no synchronisation);
+ * the exported reference-signature functions wrap them with the mirror-mode
+ * transfers.
+ */
+#ifndef SF_HOST_H
+#define SF_HOST_H
+
+#include "../sf_internal.h"
+
+/* gravity.c: PP gravity from d_mem pos/mas into d_mem acc/pot */
+int sf_solve_gravity_pp(sf_ctx* c);
+/* tree.c: bounds as src/simulation.c:245-301, build, then walk into acc/pot */
+int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number);
+/* sph.c: density, EOS and pressure force, added into acc */
+int sf_solve_sph(sf_ctx* c);
+/* simulation.c: the solver sequence of sim_solv on device state */
+int sf_solve_all(sf_ctx* c, tree_t* tree, unsigned int step_number);
+
+/* lbvh.c: SF_TREE_LBVH build (allocates *st on first use) and walk */
+typedef struct sf_lbvh sf_lbvh;
+int sf_lbvh_build(sf_lbvh** st, unsigned int N, const float* pos, const float* mas, hipStream_t s);
+int sf_lbvh_walk(sf_lbvh* st, unsigned int N, const float* pos, const float* mas, const sf_params_t* p, float* acc,
+                 float* pot, hipStream_t s);
+void sf_lbvh_destroy(sf_lbvh* st);
+
+/* initcond.c: after a host-side write of sim.mem, refresh the device copy
+   in SF_MEM_RESIDENT mode */
+int sf_after_host_write(const sim_t* sim_ptr, const char* func);
+
+#endif

@@ -1,0 +1,263 @@
+/*
+ * tree.c -- tree_init / tree_free / tree_build and solve_gravity_part_tree
+ * with the reference's signatures (src/tree.h:38-44, src/gravity.h:17).
+ *
+ * tree_t.node keeps the reference's meaning: a host array of node_t.  In
+ * SF_TREE_REFERENCE mode it receives the device build after each build (in
+ * SF_MEM_MIRROR mode); in SF_TREE_LBVH mode the device tree has a different
+ * shape and tree_t.node is not written (num_nodes reports its node count).
+ * The device side of a tree is found through the address of tree_t.node.
+ */
+#define _POSIX_C_SOURCE 200112L
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../device/tree_dev.h"
+#include "rng.h"
+#include "sf_host.h"
+
+#define SF_MAX_TREES 16
+
+struct sf_tree_dev {
+	void* key;               /* tree_t.node (host) */
+	unsigned int max_nodes;
+	sf_node* d_node;         /* reference build: max_nodes nodes */
+	float* d_bounds;         /* 6 floats */
+	unsigned int* d_meta;    /* num_nodes, status */
+	unsigned int* h_meta;    /* pinned mirror */
+	float* d_stage;          /* staging for host pos (3N) and mas (N) */
+	unsigned int stage_n;
+	hipStream_t stream;      /* for tree_build called directly */
+	sf_lbvh* lbvh;           /* SF_TREE_LBVH state (lbvh.c) */
+};
+
+static sf_tree_dev* g_tree[SF_MAX_TREES];
+
+static sf_tree_dev* tree_dev_lookup(const tree_t* t, const char* func) {
+	if (t && t->node) {
+		for (int k = 0; k < SF_MAX_TREES; ++k)
+			if (g_tree[k] && g_tree[k]->key == (void*)t->node) return g_tree[k];
+	}
+	sf_error(func, "tree_t.node %p was not allocated by tree_init()", t ? (void*)t->node : NULL);
+	return NULL;
+}
+
+static void tree_dev_destroy(sf_tree_dev* d) {
+	if (!d) return;
+	for (int k = 0; k < SF_MAX_TREES; ++k)
+		if (g_tree[k] == d) g_tree[k] = NULL;
+	if (d->stream) hipStreamSynchronize(d->stream);
+	if (d->lbvh) sf_lbvh_destroy(d->lbvh);
+	if (d->d_node) hipFree(d->d_node);
+	if (d->d_bounds) hipFree(d->d_bounds);
+	if (d->d_meta) hipFree(d->d_meta);
+	if (d->h_meta) hipHostFree(d->h_meta);
+	if (d->d_stage) hipFree(d->d_stage);
+	if (d->stream) hipStreamDestroy(d->stream);
+	free(d);
+}
+
+/* reference src/tree.c:14-55 (which memsets 0 bytes; the whole array is
+   cleared here) */
+int tree_init(tree_t* SF_RESTRICT tree_ptr, unsigned int max_nodes) {
+	if (!tree_ptr) return STARFLOOD_FAILURE;
+	tree_ptr->max_nodes = 0u;
+	tree_ptr->num_nodes = 0u;
+	for (int i = 0; i < 6; ++i) tree_ptr->bounds[i] = 0.0f;
+	tree_ptr->node = NULL;
+
+	const size_t mem_size = sizeof(node_t) * (size_t)max_nodes;
+	void* mem = NULL;
+	if (posix_memalign(&mem, (size_t)4194304u, mem_size ? mem_size : 1u) != 0 || !mem) {
+		fprintf(stderr, "%s error: mem is NULL after posix_memalign(&mem, %zu, %zu)\n", "tree_init()",
+		        (size_t)4194304u, mem_size);
+		return STARFLOOD_FAILURE;
+	}
+	memset(mem, 0, mem_size);
+
+	int slot = -1;
+	for (int k = 0; k < SF_MAX_TREES; ++k)
+		if (!g_tree[k]) { slot = k; break; }
+	sf_tree_dev* d = slot >= 0 ? (sf_tree_dev*)calloc(1, sizeof(sf_tree_dev)) : NULL;
+	if (!d) {
+		sf_error("tree_init", "too many live trees");
+		free(mem);
+		return STARFLOOD_FAILURE;
+	}
+	d->key = mem;
+	d->max_nodes = max_nodes;
+	if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess ||
+	    hipMalloc((void**)&d->d_node, (mem_size ? mem_size : 1u)) != hipSuccess ||
+	    hipMalloc((void**)&d->d_bounds, 8u * sizeof(float)) != hipSuccess ||
+	    hipMalloc((void**)&d->d_meta, 4u * sizeof(unsigned int)) != hipSuccess ||
+	    hipHostMalloc((void**)&d->h_meta, 4u * sizeof(unsigned int), hipHostMallocDefault) != hipSuccess) {
+		sf_error("tree_init", "device allocation of %u nodes failed", max_nodes);
+		tree_dev_destroy(d);
+		free(mem);
+		return STARFLOOD_FAILURE;
+	}
+	g_tree[slot] = d;
+	tree_ptr->max_nodes = max_nodes;
+	tree_ptr->node = (node_t*)mem;
+	return STARFLOOD_SUCCESS;
+}
+
+/* reference src/tree.c:57-72 */
+int tree_free(tree_t* SF_RESTRICT tree_ptr) {
+	if (!tree_ptr) return STARFLOOD_FAILURE;
+	if (tree_ptr->node) {
+		for (int k = 0; k < SF_MAX_TREES; ++k)
+			if (g_tree[k] && g_tree[k]->key == (void*)tree_ptr->node) tree_dev_destroy(g_tree[k]);
+	}
+	free(tree_ptr->node);
+	tree_ptr->max_nodes = 0u;
+	tree_ptr->num_nodes = 0u;
+	for (int i = 0; i < 6; ++i) tree_ptr->bounds[i] = 0.0f;
+	tree_ptr->node = NULL;
+	return STARFLOOD_SUCCESS;
+}
+
+static int is_device_pointer(const void* p) {
+	hipPointerAttribute_t attr;
+	memset(&attr, 0, sizeof(attr));
+	if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+		(void)hipGetLastError();
+		return 0;
+	}
+	return attr.type == hipMemoryTypeDevice;
+}
+
+/* Reference-mode build from device pos/mas into d->d_node; reports the
+   reference's errors (src/tree.c:180-189) after a synchronisation. */
+static int build_ref(sf_tree_dev* d, tree_t* t, unsigned int N, const float* pos, const float* mas, hipStream_t s) {
+	const sf_params_t* p = sf_params();
+	if (sfd_tree_build_ref(pos, mas, N, d->d_bounds, d->max_nodes, (int)p->tree_depth_max, d->d_node, d->d_meta, s))
+		return -1;
+	SF_HIP(hipMemcpyAsync(d->h_meta, d->d_meta, 2u * sizeof(unsigned int), hipMemcpyDeviceToHost, s), "tree_build");
+	SF_HIP(hipStreamSynchronize(s), "tree_build");
+	t->num_nodes = d->h_meta[0];
+	if (p->mem_mode == SF_MEM_MIRROR && t->num_nodes) {
+		SF_HIP(hipMemcpyAsync(t->node, d->d_node, (size_t)t->num_nodes * sizeof(node_t), hipMemcpyDeviceToHost, s),
+		       "tree_build");
+		SF_HIP(hipStreamSynchronize(s), "tree_build");
+	}
+	if (d->h_meta[1] == 1u) {
+		fprintf(stderr, "%s error: Exceeded maximum number of tree nodes!\n", "tree_build()");
+		return -1;
+	}
+	if (d->h_meta[1] == 2u) {
+		fprintf(stderr, "%s error: Exceeded maximum tree depth!\n", "tree_build()");
+		return -1;
+	}
+	return 0;
+}
+
+/* reference src/tree.c:74-318.  pos and mas may be host or device
+   addresses; the tree's bounds are the caller's tree_ptr->bounds. */
+int tree_build(tree_t* SF_RESTRICT tree_ptr, unsigned int num_parts, const real* SF_RESTRICT pos,
+               const real* SF_RESTRICT mas) {
+	sf_tree_dev* d = tree_dev_lookup(tree_ptr, "tree_build");
+	if (!d) return STARFLOOD_FAILURE;
+	const sf_params_t* p = sf_params();
+	hipStream_t s = d->stream;
+	const float* dpos = pos;
+	const float* dmas = mas;
+	if (num_parts && (!is_device_pointer(pos) || !is_device_pointer(mas))) {
+		if (d->stage_n < num_parts) {
+			if (d->d_stage) hipFree(d->d_stage);
+			d->d_stage = NULL;
+			d->stage_n = 0;
+			SF_HIP(hipMalloc((void**)&d->d_stage, (size_t)num_parts * 4u * sizeof(float)), "tree_build");
+			d->stage_n = num_parts;
+		}
+		SF_HIP(hipMemcpyAsync(d->d_stage, pos, (size_t)num_parts * 3u * sizeof(float), hipMemcpyHostToDevice, s),
+		       "tree_build");
+		SF_HIP(hipMemcpyAsync(d->d_stage + 3u * (size_t)num_parts, mas, (size_t)num_parts * sizeof(float),
+		                      hipMemcpyHostToDevice, s),
+		       "tree_build");
+		dpos = d->d_stage;
+		dmas = d->d_stage + 3u * (size_t)num_parts;
+	}
+	SF_HIP(hipMemcpyAsync(d->d_bounds, tree_ptr->bounds, 6u * sizeof(float), hipMemcpyHostToDevice, s), "tree_build");
+	int rc;
+	if (p->tree_mode == SF_TREE_REFERENCE) {
+		rc = build_ref(d, tree_ptr, num_parts, dpos, dmas, s);
+	} else {
+		rc = sf_lbvh_build(&d->lbvh, num_parts, dpos, dmas, s);
+		if (!rc) tree_ptr->num_nodes = num_parts ? 2u * num_parts - 1u : 0u;
+		if (hipStreamSynchronize(s) != hipSuccess) rc = -1;
+	}
+	return rc ? STARFLOOD_FAILURE : STARFLOOD_SUCCESS;
+}
+
+static void bounds_jitter(unsigned int step_number, double jitter[4]) {
+	/* src/simulation.c:281-296 */
+	uint32_t s[4] = {0x638FE17Bu + (uint32_t)step_number, 0xB10BDCE7u, 0x7B8ADB9Bu, 0xD8C5850Eu};
+	sf_pcg4d(s);
+	for (int k = 0; k < 4; ++k) jitter[k] = 2.0 * (SF_INV_U32 * (double)s[k]) - 1.0;
+}
+
+/* The tree part of src/simulation.c:245-322 on device state. */
+int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number) {
+	sf_tree_dev* d = tree_dev_lookup(tree, "solve_gravity_part_tree");
+	if (!d) return -1;
+	const sf_params_t* p = sf_params();
+	const unsigned int N = c->N;
+	hipStream_t s = c->stream;
+	const float* pos = sf_blk(c->d_mem, N, SF_E_POS);
+	const float* mas = sf_blk(c->d_mem, N, SF_E_MAS);
+	float* acc = sf_blk(c->d_mem, N, SF_E_ACC);
+	float* pot = sf_blk(c->d_mem, N, SF_E_POT);
+	const int phys = p->acc_mode == SF_ACC_PHYSICAL;
+	const float eps2 = (float)(p->epsilon * p->epsilon);
+	if (p->tree_mode == SF_TREE_REFERENCE) {
+		double jit[4];
+		bounds_jitter(step_number, jit);
+		if (N && sfd_tree_bounds_ref(pos, N, jit, d->d_bounds, s)) return -1;
+		const int rc = build_ref(d, tree, N, pos, mas, s);
+		SF_HIP(hipMemcpyAsync(tree->bounds, d->d_bounds, 6u * sizeof(float), hipMemcpyDeviceToHost, s),
+		       "solve_gravity_part_tree");
+		/* the reference walks the tree even after a failed build */
+		if (sfd_tree_walk_ref(pos, mas, N, d->d_node, d->max_nodes, 2u * 8u * p->tree_nodes_max, eps2, (float)p->theta,
+		                      (float)(-p->G), phys, p->kahan_solver, p->kahan_energy, acc, pot, s))
+			return -1;
+		return rc;
+	}
+	if (sf_lbvh_build(&d->lbvh, N, pos, mas, s)) return -1;
+	tree->num_nodes = N ? 2u * N - 1u : 0u;
+	return sf_lbvh_walk(d->lbvh, N, pos, mas, p, acc, pot, s);
+}
+
+/* reference src/gravity.c:245-663: walks the tree last built for this
+   simulation (tree_build or sim_solv) and writes acc and pot. */
+int solve_gravity_part_tree(const sim_t* SF_RESTRICT sim_ptr) {
+	if (!sim_ptr) return STARFLOOD_FAILURE;
+	sf_ctx* c = sf_ctx_lookup(sim_ptr->mem, "solve_gravity_part_tree");
+	if (!c) return STARFLOOD_FAILURE;
+	sf_tree_dev* d = tree_dev_lookup(&sim_ptr->tree, "solve_gravity_part_tree");
+	if (!d) return STARFLOOD_FAILURE;
+	const sf_params_t* p = sf_params();
+	const unsigned int N = c->N;
+	hipStream_t s = c->stream;
+	if (sf_upload_blocks(c, SF_E_POS, 3u) || sf_upload_blocks(c, SF_E_MAS, 1u)) return STARFLOOD_FAILURE;
+	/* order the tree's stream before the simulation's */
+	SF_HIP(hipStreamSynchronize(d->stream), "solve_gravity_part_tree");
+	const float* pos = sf_blk(c->d_mem, N, SF_E_POS);
+	const float* mas = sf_blk(c->d_mem, N, SF_E_MAS);
+	float* acc = sf_blk(c->d_mem, N, SF_E_ACC);
+	float* pot = sf_blk(c->d_mem, N, SF_E_POT);
+	int rc;
+	if (p->tree_mode == SF_TREE_REFERENCE) {
+		rc = sfd_tree_walk_ref(pos, mas, N, d->d_node, d->max_nodes, 2u * 8u * p->tree_nodes_max, (float)(p->epsilon * p->epsilon),
+		                       (float)p->theta, (float)(-p->G), p->acc_mode == SF_ACC_PHYSICAL, p->kahan_solver,
+		                       p->kahan_energy, acc, pot, s);
+	} else {
+		rc = d->lbvh ? sf_lbvh_walk(d->lbvh, N, pos, mas, p, acc, pot, s) : -1;
+		if (!d->lbvh) sf_error("solve_gravity_part_tree", "no tree was built");
+	}
+	if (sf_download_blocks(c, SF_E_ACC, 3u) || sf_download_blocks(c, SF_E_POT, 1u)) rc = -1;
+	if (sf_finish(c, "solve_gravity_part_tree")) rc = -1;
+	return rc ? STARFLOOD_FAILURE : STARFLOOD_SUCCESS;
+}
