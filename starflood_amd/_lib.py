@@ -116,7 +116,9 @@ def load() -> C.CDLL:
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is missing: run `python -m starflood_amd.build` "
                                "(there is no fallback implementation)")
-        lib = C.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | C.RTLD_GLOBAL)
+        # local binding: the reference's symbol names must not leak into
+        # other libraries loaded later in the process
+        lib = C.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | C.RTLD_LOCAL)
         for name, res, args in _SIG:
             fn = getattr(lib, name)
             fn.restype = res

@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -36,6 +37,19 @@ C_FLAGS = [
 ]
 
 
+INCLUDE = PKG.parent / "include"
+_PROTO = re.compile(r"^(?!typedef|#|static|\s)[\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(", re.M)
+
+
+def public_symbols() -> list[str]:
+    """Function names declared in include/*.h: the library's whole export list."""
+    names = []
+    for h in sorted(INCLUDE.glob("*.h")):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names += _PROTO.findall(text)
+    return sorted(set(names))
+
+
 def _run(cmd: list[str]) -> None:
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
@@ -60,9 +74,12 @@ def build(verbose: bool = False) -> Path:
         futs = [ex.submit(_run, cmd) for cmd, _ in jobs]
         for f in futs:
             f.result()
+    # export exactly what include/*.h declares; bind internal calls locally
+    vmap = OBJ / "exports.map"
+    vmap.write_text("{\n  global:\n" + "".join(f"    {n};\n" for n in public_symbols()) + "  local: *;\n};\n")
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *[str(o) for _, o in jobs],
-          "-Wl,--no-undefined", "-lm"])
+          "-Wl,--no-undefined", "-Wl,-Bsymbolic", f"-Wl,--version-script={vmap}", "-lm"])
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")

@@ -1,0 +1,184 @@
+"""PP gravity and the KDK step on the device through the C ABI, against the
+reference's own outputs (golden fixtures) and the oracle."""
+import hashlib
+
+import numpy as np
+import pytest
+
+from conftest import bits, golden
+from starflood_amd import _lib as L
+from starflood_amd.sim import Simulation
+
+pytestmark = pytest.mark.gpu
+
+
+def exact_sim(N, **kw):
+    args = dict(enable_tree=0, logging=0, epsilon=1e-2, dt=1e-3)
+    args.update(kw)
+    return Simulation(N, **args)
+
+
+@pytest.mark.parametrize("N", [256, 1024])
+def test_exact_pp_and_kdk_bit_identical_to_reference(N):
+    g = golden(f"f1_pp_n{N}.npz")
+    with exact_sim(N) as s:
+        s.mem[:] = g["ic"]
+        s.solve()
+        assert np.array_equal(bits(s.mem), bits(g["mem0"]))
+        for n in range(10):
+            s.step_number = n
+            s.step()
+            st = s.stats()
+            assert (st["T"], st["V"]) == tuple(g["TV"][n]), f"step {n}"
+            if n == 0:
+                assert np.array_equal(bits(s.mem), bits(g["mem1"]))
+        assert np.array_equal(bits(s.mem), bits(g["mem10"]))
+
+
+def test_config1_100_steps_bit_identical_in_step_order():
+    g = golden("f2_config1_n4096.npz")
+    with exact_sim(4096) as s:
+        s.mem[:] = g["ic"]
+        s.solve()
+        assert hashlib.sha256(s.mem.tobytes()).digest() == g["hashes"][0].tobytes()
+        for n in range(100):
+            s.step_number = n
+            s.step()
+            assert hashlib.sha256(s.mem.tobytes()).digest() == g["hashes"][n + 1].tobytes(), f"step {n}"
+        assert np.array_equal(bits(s.mem), bits(g["final"]))
+
+
+def test_exact_resident_mode_matches_mirror_mode():
+    g = golden("f1_pp_n256.npz")
+    with exact_sim(256, mem_mode=L.SF_MEM_RESIDENT) as s:
+        s.mem[:] = g["ic"]
+        s.to_device()
+        s.solve()
+        for n in range(10):
+            s.step_number = n
+            s.step()
+        s.to_host()
+        assert np.array_equal(bits(s.mem), bits(g["mem10"]))
+    L.set_params()
+
+
+@pytest.mark.parametrize("kahan", [0, 1])
+@pytest.mark.parametrize("acc_mode", [0, 1])
+def test_exact_variants_match_oracle(oracle, kahan, acc_mode):
+    rng = np.random.default_rng(10 + kahan + 2 * acc_mode)
+    N = 1000
+    pos = rng.normal(size=(N, 3)).astype(np.float32)
+    mas = rng.uniform(0.5, 1.5, N).astype(np.float32) / N
+    with exact_sim(N, kahan_solver=kahan, kahan_energy=kahan, acc_mode=acc_mode, G=0.7) as s:
+        s.pos[:] = pos
+        s.mas[:] = mas
+        s.solve_gravity_pp()
+        acc, pot = oracle.pp_ref(pos, mas, 1e-2, G=0.7, kahan_acc=kahan, kahan_pot=kahan, acc_phys=acc_mode)
+        assert np.array_equal(bits(s.acc.ravel()), bits(acc.ravel()))
+        assert np.array_equal(bits(s.pot), bits(pot))
+
+
+def test_single_particle_and_zero_mass_edge_cases(oracle):
+    with exact_sim(1) as s:
+        s.pos[:] = [[0.5, -1.0, 2.0]]
+        s.mas[:] = [3.0]
+        s.solve_gravity_pp()
+        assert np.all(s.acc == 0)
+        eps2 = np.float32(0.01 * 0.01)
+        assert s.pot[0] == np.float32(-1.0) * (np.float32(3.0) * (np.float32(1.0) / np.sqrt(eps2)))
+    with exact_sim(5, precision=L.SF_PRECISION_FAST) as s:
+        s.pos[:] = np.arange(15, dtype=np.float32).reshape(5, 3)
+        s.mas[:] = 0.0
+        s.solve_gravity_pp()
+        assert np.all(s.acc == 0) and np.all(s.pot == 0)
+    L.set_params()
+
+
+@pytest.mark.parametrize("N", [1, 100, 1000, 4097])
+def test_fast_pp_close_to_fp64_small(oracle, N):
+    rng = np.random.default_rng(N)
+    pos = rng.normal(size=(N, 3)).astype(np.float32)
+    mas = rng.uniform(0.5, 1.5, N).astype(np.float32) / N
+    with exact_sim(N, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL) as s:
+        s.pos[:] = pos
+        s.mas[:] = mas
+        s.solve_gravity_pp()
+        a64, p64 = oracle.pp_f64(pos, mas, 1e-2, acc_phys=1)
+        scale = np.linalg.norm(a64, axis=1).max() + 1e-30
+        assert np.abs(s.acc - a64).max() <= 2e-6 * scale
+        assert np.allclose(s.pot, p64, rtol=2e-6)
+    L.set_params()
+
+
+def test_f64_direct_sum_kernel_matches_cpu_oracle(oracle):
+    N = 8192
+    with exact_sim(N, epsilon=2e-3, acc_mode=L.SF_ACC_PHYSICAL) as s:
+        s.ic_plummer(seed=3, first=0, count=N)
+        idx = np.arange(0, N, 37, dtype=np.uint32)
+        a_gpu, p_gpu = s.direct_f64(idx)
+        a_cpu, p_cpu = oracle.pp_f64(s.pos.copy(), s.mas.copy(), 2e-3, acc_phys=1, idx=idx)
+        assert np.allclose(a_gpu, a_cpu, rtol=1e-11, atol=1e-14)
+        assert np.allclose(p_gpu, p_cpu, rtol=1e-12)
+    L.set_params()
+
+
+def test_fast_pp_accuracy_at_262144():
+    """The throughput kernel at the stress size against the FP64 direct sum
+    on 4096 sampled particles: relative acceleration error <= 1e-5."""
+    N = 262144
+    with exact_sim(N, epsilon=2e-3, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL,
+                   mem_mode=L.SF_MEM_RESIDENT) as s:
+        s.ic_plummer(seed=11, first=0, count=N)
+        s.solve_gravity_pp()
+        s.to_host()
+        idx = np.arange(0, N, N // 4096, dtype=np.uint32)
+        a64, p64 = s.direct_f64(idx)
+        rel = np.linalg.norm(s.acc[idx] - a64, axis=1) / np.linalg.norm(a64, axis=1)
+        print(f"rel acc error p50 {np.percentile(rel, 50):.3g} p99 {np.percentile(rel, 99):.3g} max {rel.max():.3g}")
+        assert rel.max() <= 1e-5
+        assert np.abs(s.pot[idx] / p64 - 1).max() <= 1e-6
+        # Newton's third law for equal masses: sum of accelerations ~ 0
+        assert np.linalg.norm(s.acc.astype(np.float64).sum(0)) / N <= 1e-6 * np.abs(s.acc).max()
+    L.set_params()
+
+
+def test_fast_kdk_reversible_and_deterministic():
+    N = 2048
+    with exact_sim(N, epsilon=1e-2, dt=1e-3, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL) as s:
+        s.ic_plummer(seed=5, first=0, count=N)
+        x0 = s.pos.copy()
+        v0 = s.vel.copy()
+        s.solve()
+        for n in range(20):
+            s.step()
+        x1 = s.pos.copy()
+        s.vel[:] = -s.vel
+        for n in range(20):
+            s.step()
+        assert np.abs(s.pos - x0).max() < 2e-5
+        assert np.abs(-s.vel - v0).max() < 2e-5
+    with exact_sim(N, epsilon=1e-2, dt=1e-3, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL) as s2:
+        s2.ic_plummer(seed=5, first=0, count=N)
+        s2.solve()
+        for n in range(20):
+            s2.step()
+        assert np.array_equal(bits(s2.pos), bits(x1))
+    L.set_params()
+
+
+def test_logging_writes_reference_csv_schema(tmp_path):
+    g = golden("f1_pp_n256.npz")
+    with Simulation(256, enable_tree=0, epsilon=1e-2, dt=1e-3, logging=1, out_dir=str(tmp_path)) as s:
+        s.mem[:] = g["ic"]
+        s.solve()
+        for n in range(3):
+            s.step_number = n
+            s.step()
+    lines = (tmp_path / "statistics.csv").read_text().splitlines()
+    assert lines[0] == "step_number,total_energy,potential_energy,kinetic_energy"
+    row = lines[2].split(",")
+    T, V = g["TV"][1]
+    assert row[0] == "1" and row[2] == f"{V:.015f}" and row[3] == f"{T:.015f}"
+    tl = (tmp_path / "timings_sim_step.csv").read_text().splitlines()
+    assert tl[0] == "step_number,kick_0,drift,sim_solv,kick_1,ken_sum,pen_sum" and len(tl) == 4
+    L.set_params()
