@@ -85,6 +85,7 @@ def lib() -> C.CDLL:
         L.orc_kdk_adjoint_f64.argtypes = [C.c_uint, _d, _d, _d, C.c_double, C.c_double, C.c_double, C.c_int,
                                           C.c_uint, _d, _d, _d]
         L.orc_kdk_adjoint_f64.restype = C.c_double
+        L.orc_morton63.argtypes = [C.c_uint, _f, C.POINTER(C.c_ulonglong)]
         L.orc_now.restype = C.c_double
         L.orc_num_threads.restype = C.c_int
         _lib = L
@@ -131,6 +132,14 @@ def pp_f64(pos, mas, eps, G=1.0, acc_phys=0, idx=None):
     pot = np.zeros(len(idx), np.float64)
     lib().orc_pp_f64(N, fp(pos), fp(mas), eps, G, acc_phys, up(idx), len(idx), dp(acc), dp(pot))
     return acc, pot
+
+
+def morton63(pos) -> np.ndarray:
+    pos = np.ascontiguousarray(pos, np.float32)
+    N = len(pos)
+    key = np.zeros(N, np.uint64)
+    lib().orc_morton63(N, fp(pos), key.ctypes.data_as(C.POINTER(C.c_ulonglong)))
+    return key
 
 
 def sph_ref(pos, mas, rad, acc_in, eps=1e-2, kahan=1):

@@ -418,6 +418,35 @@ int orc_sim_step(orc_sim* s) {
 	return rc;
 }
 
+void orc_morton63(unsigned N, const float* pos, unsigned long long* key) {
+	double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+	for (unsigned i = 0u; i < N; i++)
+		for (int a = 0; a < 3; a++) {
+			const double v = pos[3u * i + a];
+			lo[a] = v < lo[a] ? v : lo[a];
+			hi[a] = v > hi[a] ? v : hi[a];
+		}
+	double ext = 0.0;
+	for (int a = 0; a < 3; a++) ext = hi[a] - lo[a] > ext ? hi[a] - lo[a] : ext;
+	if (!(ext > 0.0)) ext = 1.0;
+	ext *= 1.0 + 1.0e-9;
+	const double scale = 2097152.0 / ext;
+	for (unsigned i = 0u; i < N; i++) {
+		unsigned long long q[3], k = 0ull;
+		for (int a = 0; a < 3; a++) {
+			double t = ((double)pos[3u * i + a] - lo[a]) * scale;
+			t = t < 0.0 ? 0.0 : (t > 2097151.0 ? 2097151.0 : t);
+			q[a] = (unsigned long long)t;
+		}
+		for (int b = 0; b < 21; b++) {
+			k |= ((q[0] >> b) & 1ull) << (3 * b + 2);
+			k |= ((q[1] >> b) & 1ull) << (3 * b + 1);
+			k |= ((q[2] >> b) & 1ull) << (3 * b);
+		}
+		key[i] = k;
+	}
+}
+
 void orc_pp_f64(unsigned N, const float* pos, const float* mas, double eps, double G, int acc_phys,
                 const unsigned* idx, unsigned n_idx, double* acc, double* pot) {
 	const double eps2 = eps * eps;

@@ -95,6 +95,12 @@ void orc_kdk_f64(unsigned N, double* x, double* v, const double* m, double eps, 
 double orc_kdk_adjoint_f64(unsigned N, const double* x0, const double* v0, const double* m, double eps, double G,
                            double dt, int acc_phys, unsigned nsteps, const double* target, double* gx0, double* gv0);
 
+/* 63-bit Morton codes of the SF_TREE_LBVH build, by a bit-by-bit
+   interleave (no reference counterpart): FP64 bounding box, extent = longest
+   side * (1 + 1e-9), q = floor(clamp((x - lo) * (2^21 / extent), 0,
+   2^21 - 1)), bit b of q_x at 3b + 2, of q_y at 3b + 1, of q_z at 3b. */
+void orc_morton63(unsigned N, const float* pos, unsigned long long* key);
+
 /* wall-clock seconds (CLOCK_MONOTONIC, as the reference's timing.h) */
 double orc_now(void);
 int orc_num_threads(void);

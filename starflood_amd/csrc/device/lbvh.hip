@@ -1,0 +1,613 @@
+// lbvh.hip -- the SF_TREE_LBVH Barnes-Hut tree: build and walk on gfx950.
+//
+// Build (every step, O(N), bandwidth-bound):
+//  1. FP64 bounding box (min/max are exact, so any reduction order gives the
+//     same cube).
+//  2. 63-bit Morton codes: q = floor((x - lo) * (2^21 / extent)) in FP64,
+//     clamped to [0, 2^21), bits interleaved x-y-z from the top.  Exactly
+//     reproducible by a CPU restatement.
+//  3. Stable LSD radix sort of (code, index), 8 passes of 8-bit digits, with
+//     per-tile histograms, one scan, and a ballot-based stable local rank.
+//  4. Binary radix tree from the sorted codes (Karras 2012): internal node i
+//     covers a contiguous range of the sorted particles; equal codes are
+//     separated by their sorted position.
+//  5. Monopole aggregation bottom-up with one integer arrival counter per
+//     internal node; the second child to arrive combines left and right in a
+//     fixed order, so the result does not depend on scheduling (no float
+//     atomics).
+//
+// Walk: one wavefront per group of 64 consecutive sorted particles.  Up to 64
+// pending nodes are classified at once, one per lane, against the group's
+// bounding box: accepted as a monopole when the box stays outside the node's
+// critical radius r = l/theta + |com - centre| (l = longest side of the
+// node's particle box), summed particle by particle when the node holds at
+// most leaf_size particles, opened otherwise.  Accepted cells and leaf ranges
+// are queued in LDS and evaluated 64 at a time by every lane (the same
+// 13-VALU + rsqrt pair interaction as the PP kernel), so the tree logic costs
+// ~1/64 of the interactions.  The self pair is included in leaf sums, as in
+// the reference's PP sum.
+#include <hip/hip_runtime.h>
+
+#include "../sf_internal.h"
+#include "tree_dev.h"
+
+namespace {
+
+constexpr unsigned kBlock = 256;
+constexpr unsigned kTile = 4096;          // radix sort tile (256 threads x 16)
+constexpr unsigned kRadixRounds = kTile / kBlock;
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(unsigned* __restrict__ p, unsigned n, unsigned v) {
+	const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+	if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, unsigned n) {
+	if (threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+// ---------------------------------------------------------------- bbox
+__global__ __launch_bounds__(kBlock) void k_bbox_partial(const float* __restrict__ pos, unsigned N,
+                                                         double* __restrict__ part) {
+	__shared__ double sh[6][kBlock];
+	double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+	for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+		for (int a = 0; a < 3; ++a) {
+			const double v = pos[3u * i + a];
+			lo[a] = fmin(lo[a], v);
+			hi[a] = fmax(hi[a], v);
+		}
+	}
+	for (int a = 0; a < 3; ++a) {
+		sh[a][threadIdx.x] = lo[a];
+		sh[3 + a][threadIdx.x] = hi[a];
+	}
+	__syncthreads();
+	for (unsigned h = kBlock / 2; h > 0; h >>= 1) {
+		if (threadIdx.x < h) {
+			for (int a = 0; a < 3; ++a) {
+				sh[a][threadIdx.x] = fmin(sh[a][threadIdx.x], sh[a][threadIdx.x + h]);
+				sh[3 + a][threadIdx.x] = fmax(sh[3 + a][threadIdx.x], sh[3 + a][threadIdx.x + h]);
+			}
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// box[0..2] = lo, box[3] = 2^21 / extent (extent = longest side, widened so
+// the top cell is half-open)
+__global__ void k_bbox_final(const double* __restrict__ part, unsigned nb, double* __restrict__ box) {
+	if (threadIdx.x != 0 || blockIdx.x != 0) return;
+	double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+	for (unsigned b = 0; b < nb; ++b)
+		for (int a = 0; a < 3; ++a) {
+			lo[a] = fmin(lo[a], part[b * 6 + a]);
+			hi[a] = fmax(hi[a], part[b * 6 + 3 + a]);
+		}
+	double ext = 0.0;
+	for (int a = 0; a < 3; ++a) ext = fmax(ext, hi[a] - lo[a]);
+	if (!(ext > 0.0)) ext = 1.0;
+	ext *= 1.0 + 1.0e-9;
+	for (int a = 0; a < 3; ++a) box[a] = lo[a];
+	box[3] = 2097152.0 / ext;
+}
+
+// ---------------------------------------------------------------- Morton
+__device__ __forceinline__ unsigned long long spread21(unsigned long long v) {
+	v &= 0x1FFFFFull;
+	v = (v | (v << 32)) & 0x1F00000000FFFFull;
+	v = (v | (v << 16)) & 0x1F0000FF0000FFull;
+	v = (v | (v << 8)) & 0x100F00F00F00F00Full;
+	v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
+	v = (v | (v << 2)) & 0x1249249249249249ull;
+	return v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ pos, unsigned N, unsigned n_sort,
+                                                   const double* __restrict__ box, unsigned long long* __restrict__ key,
+                                                   unsigned* __restrict__ val) {
+	const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n_sort) return;
+	if (i >= N) {
+		key[i] = ~0ull;
+		val[i] = i;
+		return;
+	}
+	const double scale = box[3];
+	unsigned long long q[3];
+	for (int a = 0; a < 3; ++a) {
+		double t = ((double)pos[3u * i + a] - box[a]) * scale;
+		t = fmin(fmax(t, 0.0), 2097151.0);
+		q[a] = (unsigned long long)t;
+	}
+	key[i] = (spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
+	val[i] = i;
+}
+
+// ---------------------------------------------------------------- radix sort
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const unsigned long long* __restrict__ key, unsigned shift,
+                                                       unsigned ntiles, unsigned* __restrict__ hist) {
+	__shared__ unsigned h[256];
+	h[threadIdx.x] = 0;
+	__syncthreads();
+	const unsigned base = blockIdx.x * kTile;
+	for (unsigned r = 0; r < kRadixRounds; ++r) {
+		const unsigned d = (unsigned)(key[base + r * kBlock + threadIdx.x] >> shift) & 255u;
+		atomicAdd(&h[d], 1u);
+	}
+	__syncthreads();
+	hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of the digit-major histogram (256 * ntiles entries) in one
+// block: each thread scans a contiguous chunk, then the chunk totals.
+__global__ __launch_bounds__(1024) void k_radix_scan(unsigned* __restrict__ hist, unsigned n) {
+	__shared__ unsigned tot[1024];
+	const unsigned chunk = (n + 1023) / 1024;
+	const unsigned b = threadIdx.x * chunk, e = min(b + chunk, n);
+	unsigned s = 0;
+	for (unsigned k = b; k < e; ++k) s += hist[k];
+	tot[threadIdx.x] = s;
+	__syncthreads();
+	for (unsigned off = 1; off < 1024; off <<= 1) {
+		const unsigned v = threadIdx.x >= off ? tot[threadIdx.x - off] : 0u;
+		__syncthreads();
+		tot[threadIdx.x] += v;
+		__syncthreads();
+	}
+	unsigned run = threadIdx.x ? tot[threadIdx.x - 1] : 0u;
+	for (unsigned k = b; k < e; ++k) {
+		const unsigned v = hist[k];
+		hist[k] = run;
+		run += v;
+	}
+}
+
+// Stable scatter: tile elements are ranked in index order; within a round of
+// 256 elements the rank among equal digits comes from a 64-lane match mask
+// built with eight ballots.
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(const unsigned long long* __restrict__ key_in,
+                                                          const unsigned* __restrict__ val_in, unsigned shift,
+                                                          unsigned ntiles, const unsigned* __restrict__ offs,
+                                                          unsigned long long* __restrict__ key_out,
+                                                          unsigned* __restrict__ val_out) {
+	__shared__ unsigned base_off[256];
+	__shared__ unsigned run[256];
+	__shared__ unsigned wcount[kBlock / 64][256];
+	const unsigned t = threadIdx.x, lane = t & 63u, w = t >> 6;
+	base_off[t] = offs[t * ntiles + blockIdx.x];
+	run[t] = 0;
+	for (unsigned q = 0; q < kBlock / 64; ++q) wcount[q][t] = 0;
+	__syncthreads();
+	const unsigned long long lt = (1ull << lane) - 1ull;
+	for (unsigned r = 0; r < kRadixRounds; ++r) {
+		const unsigned idx = blockIdx.x * kTile + r * kBlock + t;
+		const unsigned long long k = key_in[idx];
+		const unsigned v = val_in[idx];
+		const unsigned d = (unsigned)(k >> shift) & 255u;
+		unsigned long long match = ~0ull;
+		for (int bit = 0; bit < 8; ++bit) {
+			const unsigned long long b = __ballot((d >> bit) & 1u);
+			match &= ((d >> bit) & 1u) ? b : ~b;
+		}
+		const unsigned rank = __popcll(match & lt);
+		if (rank == 0) wcount[w][d] = __popcll(match);
+		__syncthreads();
+		unsigned before = run[d];
+		for (unsigned q = 0; q < w; ++q) before += wcount[q][d];
+		const unsigned dst = base_off[d] + before + rank;
+		key_out[dst] = k;
+		val_out[dst] = v;
+		__syncthreads();
+		unsigned add = 0;
+		for (unsigned q = 0; q < kBlock / 64; ++q) {
+			add += wcount[q][t];
+			wcount[q][t] = 0;
+		}
+		run[t] += add;
+		__syncthreads();
+	}
+	(void)ntiles;
+}
+
+// ---------------------------------------------------------------- tree
+__global__ __launch_bounds__(kBlock) void k_gather_sorted(const float* __restrict__ pos, const float* __restrict__ mas,
+                                                          const unsigned* __restrict__ order, unsigned N,
+                                                          float4* __restrict__ body) {
+	const unsigned p = blockIdx.x * kBlock + threadIdx.x;
+	if (p >= N) return;
+	const unsigned i = order[p];
+	body[p] = make_float4(pos[3u * i], pos[3u * i + 1u], pos[3u * i + 2u], mas[i]);
+}
+
+__device__ __forceinline__ int delta(const unsigned long long* __restrict__ key, int N, int i, int j) {
+	if (j < 0 || j >= N) return -1;
+	const unsigned long long a = key[i], b = key[j];
+	if (a != b) return __clzll((long long)(a ^ b));
+	return 64 + __clz((int)((unsigned)i ^ (unsigned)j));
+}
+
+// Internal nodes 0..N-2; node id >= N-1 is the leaf of sorted particle
+// id - (N-1).  node_lr holds (left, right), node_range (first, count).
+__global__ __launch_bounds__(kBlock) void k_karras(const unsigned long long* __restrict__ key, int N,
+                                                   int2* __restrict__ node_lr, int2* __restrict__ node_range,
+                                                   int* __restrict__ parent) {
+	const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+	if (i >= N - 1) return;
+	const int d = (delta(key, N, i, i + 1) - delta(key, N, i, i - 1)) >= 0 ? 1 : -1;
+	const int dmin = delta(key, N, i, i - d);
+	int lmax = 2;
+	while (delta(key, N, i, i + lmax * d) > dmin) lmax <<= 1;
+	int l = 0;
+	for (int t = lmax >> 1; t >= 1; t >>= 1)
+		if (delta(key, N, i, i + (l + t) * d) > dmin) l += t;
+	const int j = i + l * d;
+	const int dnode = delta(key, N, i, j);
+	int s = 0;
+	for (int div = 2;; div <<= 1) {
+		const int t = (l + div - 1) / div;
+		if (delta(key, N, i, i + (s + t) * d) > dnode) s += t;
+		if (t <= 1) break;
+	}
+	const int gamma = i + s * d + min(d, 0);
+	const int lo = min(i, j), hi = max(i, j);
+	const int left = (lo == gamma) ? (N - 1) + gamma : gamma;
+	const int right = (hi == gamma + 1) ? (N - 1) + gamma + 1 : gamma + 1;
+	node_lr[i] = make_int2(left, right);
+	node_range[i] = make_int2(lo, hi - lo + 1);
+	parent[left] = i;
+	parent[right] = i;
+	if (i == 0) parent[0] = -1;
+}
+
+// node_cm: (com x, y, z, mass); node_box: (lo x, y, z, hi x) (hi y, hi z in node_box2)
+__global__ __launch_bounds__(kBlock) void k_aggregate(const float4* __restrict__ body, int N,
+                                                      const int2* __restrict__ node_lr, const int* __restrict__ parent,
+                                                      unsigned* __restrict__ arrive, float4* __restrict__ node_cm,
+                                                      float4* __restrict__ box_lo, float4* __restrict__ box_hi) {
+	const int p = (int)(blockIdx.x * kBlock + threadIdx.x);
+	if (p >= N) return;
+	const int leaf = (N - 1) + p;
+	const float4 b = body[p];
+	node_cm[leaf] = b;
+	box_lo[leaf] = make_float4(b.x, b.y, b.z, 0.0f);
+	box_hi[leaf] = make_float4(b.x, b.y, b.z, 0.0f);
+	__threadfence();
+	int n = parent[leaf];
+	// a radix tree over 63-bit codes plus 32-bit positions is at most 96
+	// levels deep; the step cap only guards against a malformed tree
+	for (int step = 0; n >= 0 && n < N - 1 && step < 128; ++step) {
+		const unsigned old = atomicAdd(&arrive[n], 1u);
+		if (old != 1u) return;        // first arrival: the other child is not done yet
+		__threadfence();
+		const int2 lr = node_lr[n];
+		// the agent-scope fences around the arrival counter make the sibling's
+		// writes (possibly from another XCD) visible to these loads
+		const float4 a = node_cm[lr.x];
+		const float4 c = node_cm[lr.y];
+		const float4 alo = box_lo[lr.x];
+		const float4 clo = box_lo[lr.y];
+		const float4 ahi = box_hi[lr.x];
+		const float4 chi = box_hi[lr.y];
+		const float m = a.w + c.w;
+		float4 cm;
+		if (m > 0.0f) {
+			const float inv = 1.0f / m;
+			cm = make_float4((a.w * a.x + c.w * c.x) * inv, (a.w * a.y + c.w * c.y) * inv,
+			                 (a.w * a.z + c.w * c.z) * inv, m);
+		} else {
+			cm = make_float4(0.5f * (a.x + c.x), 0.5f * (a.y + c.y), 0.5f * (a.z + c.z), 0.0f);
+		}
+		node_cm[n] = cm;
+		box_lo[n] = make_float4(fminf(alo.x, clo.x), fminf(alo.y, clo.y), fminf(alo.z, clo.z), 0.0f);
+		box_hi[n] = make_float4(fmaxf(ahi.x, chi.x), fmaxf(ahi.y, chi.y), fmaxf(ahi.z, chi.z), 0.0f);
+		__threadfence();
+		n = parent[n];
+	}
+}
+
+// Walk records: rec_a = (com, mass), rec_b = (r_crit^2, a, b, count) where
+// (a, b) = (first, count) for nodes of at most leaf_size particles and
+// (left, right) otherwise; leaves of one particle carry count 1.
+__global__ __launch_bounds__(kBlock) void k_records(int N, const int2* __restrict__ node_lr,
+                                                    const int2* __restrict__ node_range,
+                                                    const float4* __restrict__ node_cm,
+                                                    const float4* __restrict__ box_lo, const float4* __restrict__ box_hi,
+                                                    float inv_theta, int leaf_size, float4* __restrict__ rec_a,
+                                                    int4* __restrict__ rec_b) {
+	const int n = (int)(blockIdx.x * kBlock + threadIdx.x);
+	if (n >= 2 * N - 1) return;
+	const float4 cm = node_cm[n];
+	rec_a[n] = cm;
+	int first, count, l = -1, r = -1;
+	if (n < N - 1) {
+		const int2 rg = node_range[n];
+		first = rg.x;
+		count = rg.y;
+		l = node_lr[n].x;
+		r = node_lr[n].y;
+	} else {
+		first = n - (N - 1);
+		count = 1;
+	}
+	const float4 lo = box_lo[n], hi = box_hi[n];
+	const float sx = hi.x - lo.x, sy = hi.y - lo.y, sz = hi.z - lo.z;
+	const float side = fmaxf(sx, fmaxf(sy, sz));
+	const float ox = cm.x - 0.5f * (lo.x + hi.x), oy = cm.y - 0.5f * (lo.y + hi.y), oz = cm.z - 0.5f * (lo.z + hi.z);
+	const float off = sqrtf(ox * ox + oy * oy + oz * oz);
+	const float rc = side * inv_theta + off;
+	const bool leafy = count <= leaf_size;
+	// w carries the particle count, negated for nodes that are opened into
+	// their children (whose ids are then in y, z)
+	rec_b[n] = make_int4(__float_as_int(rc * rc), leafy ? first : l, leafy ? count : r, leafy ? count : -count);
+}
+
+// ---------------------------------------------------------------- walk
+constexpr int kStack = 2048;       // node ids per wavefront
+constexpr int kHead = 160;         // room for a depth-first descent
+constexpr int kList = 128;         // queued cells / leaves per wavefront
+constexpr int kWaves = kBlock / 64;
+
+__device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+__device__ __forceinline__ float wave_min(float v) {
+	for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+	return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+	for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+	return v;
+}
+
+struct Acc {
+	float tx, ty, tz, tp;   // tile partial sums
+	float sx, sy, sz, sp;   // running sums
+	float cx, cy, cz, cp;   // Kahan compensations
+};
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
+	const float y = x - c;
+	const float t = s + y;
+	c = (t - s) - y;
+	s = t;
+}
+
+__device__ __forceinline__ void fold(Acc& a) {
+	kahan_add(a.sx, a.cx, a.tx);
+	kahan_add(a.sy, a.cy, a.ty);
+	kahan_add(a.sz, a.cz, a.tz);
+	kahan_add(a.sp, a.cp, a.tp);
+	a.tx = a.ty = a.tz = a.tp = 0.0f;
+}
+
+__device__ __forceinline__ void interact(Acc& a, float xi, float yi, float zi, float4 bj, float eps2) {
+	const float dx = bj.x - xi, dy = bj.y - yi, dz = bj.z - zi;
+	float r2 = __builtin_fmaf(dx, dx, eps2);
+	r2 = __builtin_fmaf(dy, dy, r2);
+	r2 = __builtin_fmaf(dz, dz, r2);
+	const float inv = __builtin_amdgcn_rsqf(r2);
+	const float mi = bj.w * inv;
+	const float w = mi * (inv * inv);
+	a.tx = __builtin_fmaf(w, dx, a.tx);
+	a.ty = __builtin_fmaf(w, dy, a.ty);
+	a.tz = __builtin_fmaf(w, dz, a.tz);
+	a.tp += mi;
+}
+
+__global__ __launch_bounds__(kBlock) void k_walk(const float4* __restrict__ body, const unsigned* __restrict__ order,
+                                                 int N, const float4* __restrict__ rec_a,
+                                                 const int4* __restrict__ rec_b, float eps2, double G, int acc_phys,
+                                                 float* __restrict__ acc, float* __restrict__ pot,
+                                                 unsigned long long* __restrict__ counters) {
+	__shared__ int s_stack[kWaves][kStack];
+	__shared__ float4 s_cell[kWaves][kList];
+	__shared__ int2 s_leaf[kWaves][kList];
+	const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+	const int group = (int)(blockIdx.x * kWaves + w);
+	const int base = group * 64;
+	if (base >= N) return;   // whole wavefront leaves together
+	const int p = base + (int)lane;
+	const bool live = p < N;
+	const float4 me = body[live ? p : N - 1];
+	const float xi = me.x, yi = me.y, zi = me.z;
+	const float gx0 = wave_min(xi), gy0 = wave_min(yi), gz0 = wave_min(zi);
+	const float gx1 = wave_max(xi), gy1 = wave_max(yi), gz1 = wave_max(zi);
+	int* stack = s_stack[w];
+	float4* cells = s_cell[w];
+	int2* leaves = s_leaf[w];
+	const unsigned long long lt = (1ull << lane) - 1ull;
+
+	Acc a;
+	a.tx = a.ty = a.tz = a.tp = a.sx = a.sy = a.sz = a.sp = a.cx = a.cy = a.cz = a.cp = 0.0f;
+	unsigned long long n_cell = 0, n_part = 0;
+	const int n_nodes = 2 * N - 1;
+	const unsigned budget = 2u * (unsigned)n_nodes + 64u;
+	unsigned popped = 0, bad = 0;
+	int sp = 0, nc = 0, nl = 0;
+	if (lane == 0) stack[0] = 0;   // root (internal node 0, or the single leaf when N == 1)
+	sp = 1;
+	wave_fence();
+
+	while (sp > 0 || nc > 0 || nl > 0) {
+		if (sp > 0) {
+			const int k = (sp + 64 <= kStack - kHead) ? min(64, sp) : 1;
+			int node = -1;
+			if ((int)lane < k) node = stack[sp - 1 - (int)lane];
+			sp -= k;
+			wave_fence();
+			popped += (unsigned)k;
+			int cls = 0;   // 0 none, 1 cell, 2 leaf, 3 open
+			float4 ra = make_float4(0.f, 0.f, 0.f, 0.f);
+			int4 rb = make_int4(0, 0, 0, 0);
+			if (node >= n_nodes) {
+				node = -1;   // only a malformed tree gets here
+				bad = 1u;
+			}
+			if (node >= 0) {
+				ra = rec_a[node];
+				rb = rec_b[node];
+				const float dx = fmaxf(fmaxf(gx0 - ra.x, ra.x - gx1), 0.0f);
+				const float dy = fmaxf(fmaxf(gy0 - ra.y, ra.y - gy1), 0.0f);
+				const float dz = fmaxf(fmaxf(gz0 - ra.z, ra.z - gz1), 0.0f);
+				const float d2 = dx * dx + dy * dy + dz * dz;
+				const int count = abs(rb.w);
+				if (d2 > __int_as_float(rb.x) && count > 1) cls = 1;
+				else if (rb.w > 0) cls = 2;   // (first, count) record
+				else cls = 3;
+			}
+			const unsigned long long m_cell = __ballot(cls == 1);
+			const unsigned long long m_leaf = __ballot(cls == 2);
+			const unsigned long long m_open = __ballot(cls == 3);
+			if (cls == 1) cells[nc + __popcll(m_cell & lt)] = ra;
+			if (cls == 2) leaves[nl + __popcll(m_leaf & lt)] = make_int2(rb.y, rb.z);
+			if (cls == 3) {
+				const int at = sp + 2 * __popcll(m_open & lt);
+				if (at + 1 < kStack) {   // always true by the depth argument above
+					stack[at] = rb.y;
+					stack[at + 1] = rb.z;
+				} else {
+					bad = 1u;
+				}
+			}
+			nc += __popcll(m_cell);
+			nl += __popcll(m_leaf);
+			sp = min(sp + 2 * __popcll(m_open), kStack);
+			wave_fence();
+			// a tree visits each node at most once per group
+			if (popped > budget) {
+				bad = 1u;
+				sp = 0;
+			}
+		}
+		const bool drain = sp == 0;
+		if (nc >= 64 || (drain && nc > 0)) {
+			const int take = min(nc, 64);
+			for (int q = 0; q < take; ++q) interact(a, xi, yi, zi, cells[q], eps2);
+			fold(a);
+			n_cell += (unsigned long long)take;
+			wave_fence();
+			if ((int)lane < nc - take) cells[lane] = cells[take + lane];
+			nc -= take;
+			wave_fence();
+		}
+		if (nl >= 64 || (drain && nl > 0)) {
+			const int take = min(nl, 64);
+			for (int q = 0; q < take; ++q) {
+				const int2 lf = leaves[q];
+				for (int j = lf.x; j < lf.x + lf.y; ++j) interact(a, xi, yi, zi, body[j], eps2);
+				n_part += (unsigned long long)lf.y;
+				fold(a);
+			}
+			wave_fence();
+			if ((int)lane < nl - take) leaves[lane] = leaves[take + lane];
+			nl -= take;
+			wave_fence();
+		}
+	}
+	if (live) {
+		const unsigned i = order[p];
+		const double sx = (double)a.sx - (double)a.cx, sy = (double)a.sy - (double)a.cy,
+		             sz = (double)a.sz - (double)a.cz, sp2 = (double)a.sp - (double)a.cp;
+		const double f = acc_phys ? G : G * (double)me.w;
+		acc[3u * i + 0u] = (float)(f * sx);
+		acc[3u * i + 1u] = (float)(f * sy);
+		acc[3u * i + 2u] = (float)(f * sz);
+		pot[i] = (float)(-G * sp2);
+	}
+	const bool any_bad = __ballot(bad != 0u) != 0ull;
+	if (counters && lane == 0) {
+		const unsigned long long nlive = (unsigned long long)min(64, N - base);
+		atomicAdd(&counters[0], n_cell * nlive);
+		atomicAdd(&counters[1], n_part * nlive);
+		if (any_bad) atomicAdd(&counters[2], 1ull);
+	}
+}
+
+}  // namespace
+
+extern "C" {
+
+unsigned sfd_lbvh_bbox_blocks(void) { return 256u; }
+
+int sfd_lbvh_keys(const float* pos, unsigned N, unsigned n_sort, double* bbox_part, double* box,
+                  unsigned long long* key, unsigned* val, hipStream_t s) {
+	const unsigned nb = sfd_lbvh_bbox_blocks();
+	if (N == 0 || n_sort % kTile != 0) return -1;
+	hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(kBlock), 0, s, pos, N, bbox_part);
+	hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, bbox_part, nb, box);
+	hipLaunchKernelGGL(k_morton, dim3((n_sort + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pos, N, n_sort, box, key,
+	                   val);
+	return sf_hip_check(hipGetLastError(), "sfd_lbvh_keys", "launch");
+}
+
+unsigned sfd_radix_tile(void) { return kTile; }
+
+// Sorts n_sort (a multiple of the tile) pairs; the result ends in (key0,
+// val0) after the eight passes.
+int sfd_radix_sort(unsigned long long* key0, unsigned* val0, unsigned long long* key1, unsigned* val1,
+                   unsigned n_sort, unsigned* hist, hipStream_t s) {
+	const unsigned ntiles = n_sort / kTile;
+	for (unsigned pass = 0; pass < 8; ++pass) {
+		const unsigned shift = 8u * pass;
+		unsigned long long* kin = (pass & 1) ? key1 : key0;
+		unsigned* vin = (pass & 1) ? val1 : val0;
+		unsigned long long* kout = (pass & 1) ? key0 : key1;
+		unsigned* vout = (pass & 1) ? val0 : val1;
+		hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, kin, shift, ntiles, hist);
+		hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, hist, 256u * ntiles);
+		hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, shift, ntiles, hist, kout, vout);
+	}
+	return sf_hip_check(hipGetLastError(), "sfd_radix_sort", "launch");
+}
+
+int sfd_lbvh_topology(const float* pos, const float* mas, const unsigned long long* key, const unsigned* order,
+                      unsigned N, void* body, void* node_lr, void* node_range, int* parent, unsigned* arrive,
+                      void* node_cm, void* box_lo, void* box_hi, hipStream_t s) {
+	const unsigned g = (N + kBlock - 1) / kBlock;
+	float4* b4 = static_cast<float4*>(body);
+	int2* lr = static_cast<int2*>(node_lr);
+	int2* rg = static_cast<int2*>(node_range);
+	const int tok = sf_prof_begin("lbvh_topology", s);
+	hipLaunchKernelGGL(k_gather_sorted, dim3(g), dim3(kBlock), 0, s, pos, mas, order, N, b4);
+	if (N > 1) {
+		hipLaunchKernelGGL(k_karras, dim3((N - 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, key, (int)N, lr, rg,
+		                   parent);
+		hipLaunchKernelGGL(k_fill_u32, dim3((N - 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, arrive, N - 1, 0u);
+	} else {
+		hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, s, reinterpret_cast<unsigned*>(parent), 1u,
+		                   0xFFFFFFFFu);
+	}
+	hipLaunchKernelGGL(k_aggregate, dim3(g), dim3(kBlock), 0, s, b4, (int)N, lr, parent, arrive,
+	                   static_cast<float4*>(node_cm), static_cast<float4*>(box_lo), static_cast<float4*>(box_hi));
+	sf_prof_end(tok, s);
+	return sf_hip_check(hipGetLastError(), "sfd_lbvh_topology", "launch");
+}
+
+int sfd_lbvh_records(unsigned N, const void* node_lr, const void* node_range, const void* node_cm, const void* box_lo,
+                     const void* box_hi, float theta, int leaf_size, void* rec_a, void* rec_b, hipStream_t s) {
+	const unsigned nn = 2u * N - 1u;
+	const float inv_theta = theta > 0.0f ? 1.0f / theta : 3.0e38f;
+	hipLaunchKernelGGL(k_records, dim3((nn + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (int)N,
+	                   static_cast<const int2*>(node_lr), static_cast<const int2*>(node_range),
+	                   static_cast<const float4*>(node_cm), static_cast<const float4*>(box_lo),
+	                   static_cast<const float4*>(box_hi), inv_theta, leaf_size, static_cast<float4*>(rec_a),
+	                   static_cast<int4*>(rec_b));
+	return sf_hip_check(hipGetLastError(), "sfd_lbvh_records", "launch");
+}
+
+int sfd_lbvh_walk(const void* body, const unsigned* order, unsigned N, const void* rec_a, const void* rec_b,
+                  float eps2, double G, int acc_phys, float* acc, float* pot, unsigned long long* counters,
+                  hipStream_t s) {
+	if (N == 0) return 0;
+	const unsigned groups = (N + 63) / 64;
+	if (counters) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, counters, 4u);
+	const int tok = sf_prof_begin("lbvh_walk", s);
+	hipLaunchKernelGGL(k_walk, dim3((groups + kWaves - 1) / kWaves), dim3(kBlock), 0, s,
+	                   static_cast<const float4*>(body), order, (int)N, static_cast<const float4*>(rec_a),
+	                   static_cast<const int4*>(rec_b), eps2, G, acc_phys, acc, pot, counters);
+	sf_prof_end(tok, s);
+	return sf_hip_check(hipGetLastError(), "sfd_lbvh_walk", "launch");
+}
+
+}  // extern "C"

@@ -31,6 +31,29 @@ int sfd_tree_walk_ref(const float* pos, const float* mas, unsigned int N, const 
                       float eps2, float theta, float negG, int acc_phys, int kahan_acc, int kahan_pot, float* acc,
                       float* pot, hipStream_t s);
 
+/* --- LBVH tree (lbvh.hip); node arrays are float4 / int2 / int4 on the
+   device, passed untyped through C --- */
+unsigned int sfd_lbvh_bbox_blocks(void);
+unsigned int sfd_radix_tile(void);
+/* bbox (FP64), then 63-bit Morton keys and identity values for n_sort
+   (a multiple of the radix tile) entries; padding keys are all ones */
+int sfd_lbvh_keys(const float* pos, unsigned int N, unsigned int n_sort, double* bbox_part, double* box,
+                  unsigned long long* key, unsigned int* val, hipStream_t s);
+/* stable 8-pass LSD radix sort; the result is in (key0, val0) */
+int sfd_radix_sort(unsigned long long* key0, unsigned int* val0, unsigned long long* key1, unsigned int* val1,
+                   unsigned int n_sort, unsigned int* hist, hipStream_t s);
+int sfd_lbvh_topology(const float* pos, const float* mas, const unsigned long long* key, const unsigned int* order,
+                      unsigned int N, void* body, void* node_lr, void* node_range, int* parent, unsigned int* arrive,
+                      void* node_cm, void* box_lo, void* box_hi, hipStream_t s);
+int sfd_lbvh_records(unsigned int N, const void* node_lr, const void* node_range, const void* node_cm,
+                     const void* box_lo, const void* box_hi, float theta, int leaf_size, void* rec_a, void* rec_b,
+                     hipStream_t s);
+/* counters (4 x u64, zeroed here): cell interactions, particle
+   interactions, groups that met a malformed tree */
+int sfd_lbvh_walk(const void* body, const unsigned int* order, unsigned int N, const void* rec_a, const void* rec_b,
+                  float eps2, double G, int acc_phys, float* acc, float* pot, unsigned long long* counters,
+                  hipStream_t s);
+
 #ifdef __cplusplus
 }
 #endif

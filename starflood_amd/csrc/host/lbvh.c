@@ -1,25 +1,170 @@
 /*
- * lbvh.c -- host side of the SF_TREE_LBVH Barnes-Hut tree.
+ * lbvh.c -- host side of the SF_TREE_LBVH Barnes-Hut tree: device buffers
+ * sized for N, the build sequence and the walk (kernels in device/lbvh.hip).
  */
 #include <stdlib.h>
+#include <string.h>
 
+#include "../device/tree_dev.h"
 #include "sf_host.h"
 
 struct sf_lbvh {
-	unsigned int n_alloc;
+	unsigned int n_alloc;     /* N the buffers are sized for */
+	unsigned int N;           /* particles in the last build */
+	unsigned int n_sort;
+	unsigned long long* key0;
+	unsigned long long* key1;
+	unsigned int* val0;
+	unsigned int* val1;
+	unsigned int* hist;
+	double* bbox_part;
+	double* box;
+	void* body;               /* N float4, sorted (x, y, z, m) */
+	void* node_lr;            /* N-1 int2 */
+	void* node_range;         /* N-1 int2 */
+	int* parent;              /* 2N-1 */
+	unsigned int* arrive;     /* N-1 */
+	void* node_cm;            /* 2N-1 float4 */
+	void* box_lo;             /* 2N-1 float4 */
+	void* box_hi;             /* 2N-1 float4 */
+	void* rec_a;              /* 2N-1 float4 */
+	void* rec_b;              /* 2N-1 int4 */
+	unsigned long long* counters;   /* 4 */
+	unsigned long long* h_counters; /* pinned */
 };
 
+static void free_buffers(sf_lbvh* t) {
+	void* ptrs[] = {t->key0, t->key1, t->val0, t->val1, t->hist, t->bbox_part, t->box, t->body, t->node_lr,
+	                t->node_range, t->parent, t->arrive, t->node_cm, t->box_lo, t->box_hi, t->rec_a, t->rec_b,
+	                t->counters};
+	for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k)
+		if (ptrs[k]) hipFree(ptrs[k]);
+	if (t->h_counters) hipHostFree(t->h_counters);
+	memset(t, 0, sizeof(*t));
+}
+
+void sf_lbvh_destroy(sf_lbvh* t) {
+	if (!t) return;
+	hipDeviceSynchronize();
+	free_buffers(t);
+	free(t);
+}
+
+static int alloc(void** p, size_t bytes) { return hipMalloc(p, bytes ? bytes : 16u) == hipSuccess ? 0 : -1; }
+
+static int ensure(sf_lbvh** st, unsigned int N) {
+	if (!*st) {
+		*st = (sf_lbvh*)calloc(1, sizeof(sf_lbvh));
+		if (!*st) return -1;
+	}
+	sf_lbvh* t = *st;
+	if (t->n_alloc >= N && t->key0) return 0;
+	hipDeviceSynchronize();
+	free_buffers(t);
+	const unsigned int tile = sfd_radix_tile();
+	const size_t n_sort = ((size_t)N + tile - 1u) / tile * tile;
+	const size_t nn = 2u * (size_t)N;   /* >= 2N-1 */
+	int rc = 0;
+	rc |= alloc((void**)&t->key0, n_sort * 8u);
+	rc |= alloc((void**)&t->key1, n_sort * 8u);
+	rc |= alloc((void**)&t->val0, n_sort * 4u);
+	rc |= alloc((void**)&t->val1, n_sort * 4u);
+	rc |= alloc((void**)&t->hist, 256u * (n_sort / tile) * 4u);
+	rc |= alloc((void**)&t->bbox_part, sfd_lbvh_bbox_blocks() * 6u * 8u);
+	rc |= alloc((void**)&t->box, 4u * 8u);
+	rc |= alloc(&t->body, (size_t)N * 16u);
+	rc |= alloc(&t->node_lr, (size_t)N * 8u);
+	rc |= alloc(&t->node_range, (size_t)N * 8u);
+	rc |= alloc((void**)&t->parent, nn * 4u);
+	rc |= alloc((void**)&t->arrive, (size_t)N * 4u);
+	rc |= alloc(&t->node_cm, nn * 16u);
+	rc |= alloc(&t->box_lo, nn * 16u);
+	rc |= alloc(&t->box_hi, nn * 16u);
+	rc |= alloc(&t->rec_a, nn * 16u);
+	rc |= alloc(&t->rec_b, nn * 16u);
+	rc |= alloc((void**)&t->counters, 4u * 8u);
+	if (!rc && hipHostMalloc((void**)&t->h_counters, 4u * 8u, hipHostMallocDefault) != hipSuccess) rc = -1;
+	if (rc) {
+		sf_error("tree_build", "device allocation for an LBVH of %u particles failed", N);
+		free_buffers(t);
+		return -1;
+	}
+	t->n_alloc = N;
+	t->n_sort = (unsigned int)n_sort;
+	return 0;
+}
+
 int sf_lbvh_build(sf_lbvh** st, unsigned int N, const float* pos, const float* mas, hipStream_t s) {
-	(void)st; (void)N; (void)pos; (void)mas; (void)s;
-	sf_error("tree_build", "SF_TREE_LBVH is not built in this library version");
-	return -1;
+	if (N == 0) {
+		if (*st) (*st)->N = 0;
+		return 0;
+	}
+	if (ensure(st, N)) return -1;
+	sf_lbvh* t = *st;
+	const unsigned int tile = sfd_radix_tile();
+	t->n_sort = (N + tile - 1u) / tile * tile;
+	t->N = N;
+	const int tok = sf_prof_begin("lbvh_build", s);
+	if (sfd_lbvh_keys(pos, N, t->n_sort, t->bbox_part, t->box, t->key0, t->val0, s)) return -1;
+	if (sfd_radix_sort(t->key0, t->val0, t->key1, t->val1, t->n_sort, t->hist, s)) return -1;
+	if (sfd_lbvh_topology(pos, mas, t->key0, t->val0, N, t->body, t->node_lr, t->node_range, t->parent, t->arrive,
+	                      t->node_cm, t->box_lo, t->box_hi, s))
+		return -1;
+	sf_prof_end(tok, s);
+	return 0;
 }
 
-int sf_lbvh_walk(sf_lbvh* st, unsigned int N, const float* pos, const float* mas, const sf_params_t* p, float* acc,
+int sf_lbvh_walk(sf_lbvh* t, unsigned int N, const float* pos, const float* mas, const sf_params_t* p, float* acc,
                  float* pot, hipStream_t s) {
-	(void)st; (void)N; (void)pos; (void)mas; (void)p; (void)acc; (void)pot; (void)s;
-	sf_error("solve_gravity_part_tree", "SF_TREE_LBVH is not built in this library version");
-	return -1;
+	(void)pos;
+	(void)mas;
+	if (N == 0) return 0;
+	if (!t || t->N != N) {
+		sf_error("solve_gravity_part_tree", "the tree was built for %u particles, not %u", t ? t->N : 0u, N);
+		return -1;
+	}
+	if (sfd_lbvh_records(N, t->node_lr, t->node_range, t->node_cm, t->box_lo, t->box_hi, (float)p->theta,
+	                     (int)p->leaf_size, t->rec_a, t->rec_b, s))
+		return -1;
+	return sfd_lbvh_walk(t->body, t->val0, N, t->rec_a, t->rec_b, (float)(p->epsilon * p->epsilon), p->G,
+	                     p->acc_mode == SF_ACC_PHYSICAL, acc, pot, t->counters, s);
 }
 
-void sf_lbvh_destroy(sf_lbvh* st) { free(st); }
+int sf_lbvh_export_impl(sf_lbvh* t, hipStream_t s, unsigned long long* keys, unsigned int* order,
+                        unsigned long long counts[3], float* node_cm, int* node_lr_range) {
+	if (!t || t->N == 0) {
+		sf_error("sf_lbvh_export", "no LBVH has been built");
+		return -1;
+	}
+	const size_t N = t->N;
+	SF_HIP(hipStreamSynchronize(s), "sf_lbvh_export");
+	if (keys) SF_HIP(hipMemcpy(keys, t->key0, N * 8u, hipMemcpyDeviceToHost), "sf_lbvh_export");
+	if (order) SF_HIP(hipMemcpy(order, t->val0, N * 4u, hipMemcpyDeviceToHost), "sf_lbvh_export");
+	if (counts) {
+		SF_HIP(hipMemcpy(t->h_counters, t->counters, 4u * 8u, hipMemcpyDeviceToHost), "sf_lbvh_export");
+		for (int k = 0; k < 3; ++k) counts[k] = t->h_counters[k];
+	}
+	if (node_cm) SF_HIP(hipMemcpy(node_cm, t->node_cm, (2u * N - 1u) * 16u, hipMemcpyDeviceToHost), "sf_lbvh_export");
+	if (node_lr_range && N > 1) {
+		int* tmp = (int*)malloc((N - 1u) * 8u);
+		if (!tmp) return -1;
+		if (hipMemcpy(tmp, t->node_lr, (N - 1u) * 8u, hipMemcpyDeviceToHost) != hipSuccess) {
+			free(tmp);
+			return -1;
+		}
+		for (size_t i = 0; i + 1 < N; ++i) {
+			node_lr_range[4 * i + 0] = tmp[2 * i + 0];
+			node_lr_range[4 * i + 1] = tmp[2 * i + 1];
+		}
+		if (hipMemcpy(tmp, t->node_range, (N - 1u) * 8u, hipMemcpyDeviceToHost) != hipSuccess) {
+			free(tmp);
+			return -1;
+		}
+		for (size_t i = 0; i + 1 < N; ++i) {
+			node_lr_range[4 * i + 2] = tmp[2 * i + 0];
+			node_lr_range[4 * i + 3] = tmp[2 * i + 1];
+		}
+		free(tmp);
+	}
+	return 0;
+}

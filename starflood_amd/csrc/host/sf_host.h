@@ -13,6 +13,9 @@
 int sf_solve_gravity_pp(sf_ctx* c);
 /* tree.c: bounds as src/simulation.c:245-301, build, then walk into acc/pot */
 int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number);
+/* tree.c: after an LBVH solve on stream s, fail if the walk met a malformed
+   tree (synchronises s; no-op in reference mode) */
+int sf_tree_check(tree_t* tree, hipStream_t s);
 /* sph.c: density, EOS and pressure force, added into acc */
 int sf_solve_sph(sf_ctx* c);
 /* simulation.c: the solver sequence of sim_solv on device state */
@@ -24,6 +27,9 @@ int sf_lbvh_build(sf_lbvh** st, unsigned int N, const float* pos, const float* m
 int sf_lbvh_walk(sf_lbvh* st, unsigned int N, const float* pos, const float* mas, const sf_params_t* p, float* acc,
                  float* pot, hipStream_t s);
 void sf_lbvh_destroy(sf_lbvh* st);
+/* synchronises s, then copies out what is asked for (NULL: skip) */
+int sf_lbvh_export_impl(sf_lbvh* st, hipStream_t s, unsigned long long* keys, unsigned int* order,
+                        unsigned long long counts[3], float* node_cm, int* node_lr_range);
 
 /* initcond.c: after a host-side write of sim.mem, refresh the device copy
    in SF_MEM_RESIDENT mode */

@@ -165,6 +165,7 @@ int sim_solv(sim_t* SF_RESTRICT sim_ptr) {
 	if (sf_solve_all(c, &sim_ptr->tree, sim_ptr->step_number)) rc = -1;
 	if (download_solver_outputs(c)) rc = -1;
 	if (sf_finish(c, "sim_solv")) rc = -1;
+	if (sf_params()->enable_tree && sf_tree_check(&sim_ptr->tree, c->stream)) rc = -1;
 	return rc ? STARFLOOD_FAILURE : STARFLOOD_SUCCESS;
 }
 

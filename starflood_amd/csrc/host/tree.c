@@ -230,6 +230,33 @@ int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number) {
 	return sf_lbvh_walk(d->lbvh, N, pos, mas, p, acc, pot, s);
 }
 
+int sf_lbvh_export(const tree_t* tree_ptr, unsigned long long* keys, unsigned int* order,
+                   unsigned long long counts[3], float* node_cm, int* node_lr_range) {
+	sf_tree_dev* d = tree_dev_lookup(tree_ptr, "sf_lbvh_export");
+	if (!d) return STARFLOOD_FAILURE;
+	SF_HIP(hipDeviceSynchronize(), "sf_lbvh_export");
+	return sf_lbvh_export_impl(d->lbvh, d->stream, keys, order, counts, node_cm, node_lr_range) ? STARFLOOD_FAILURE
+	                                                                                           : STARFLOOD_SUCCESS;
+}
+
+/* After an LBVH walk on stream s: fail if any wavefront met a malformed
+   tree (synchronises s). */
+static int lbvh_check(sf_tree_dev* d, hipStream_t s) {
+	unsigned long long counts[3] = {0, 0, 0};
+	if (!d->lbvh || sf_lbvh_export_impl(d->lbvh, s, NULL, NULL, counts, NULL, NULL)) return -1;
+	if (counts[2]) {
+		sf_error("solve_gravity_part_tree", "%llu wavefront groups met a malformed tree", counts[2]);
+		return -1;
+	}
+	return 0;
+}
+
+int sf_tree_check(tree_t* tree, hipStream_t s) {
+	if (sf_params()->tree_mode != SF_TREE_LBVH) return 0;
+	sf_tree_dev* d = tree_dev_lookup(tree, "sim_solv");
+	return d ? lbvh_check(d, s) : -1;
+}
+
 /* reference src/gravity.c:245-663: walks the tree last built for this
    simulation (tree_build or sim_solv) and writes acc and pot. */
 int solve_gravity_part_tree(const sim_t* SF_RESTRICT sim_ptr) {
@@ -256,6 +283,7 @@ int solve_gravity_part_tree(const sim_t* SF_RESTRICT sim_ptr) {
 	} else {
 		rc = d->lbvh ? sf_lbvh_walk(d->lbvh, N, pos, mas, p, acc, pot, s) : -1;
 		if (!d->lbvh) sf_error("solve_gravity_part_tree", "no tree was built");
+		if (!rc && lbvh_check(d, s)) rc = -1;
 	}
 	if (sf_download_blocks(c, SF_E_ACC, 3u) || sf_download_blocks(c, SF_E_POT, 1u)) rc = -1;
 	if (sf_finish(c, "solve_gravity_part_tree")) rc = -1;

@@ -1,0 +1,124 @@
+"""SF_TREE_LBVH Barnes-Hut on the device: bit-exact Morton codes and stable
+sort against the oracle, tree invariants, the theta -> 0 limit equal to the
+direct sum, and the force error against the FP64 direct sum."""
+import numpy as np
+import pytest
+
+from conftest import bits
+from starflood_amd import _lib as L
+from starflood_amd.sim import Simulation
+
+pytestmark = pytest.mark.gpu
+
+
+def lbvh_sim(N, **kw):
+    args = dict(enable_tree=1, tree_mode=L.SF_TREE_LBVH, logging=0, epsilon=1e-2, dt=1e-3,
+                acc_mode=L.SF_ACC_PHYSICAL, precision=L.SF_PRECISION_FAST)
+    args.update(kw)
+    return Simulation(N, **args)
+
+
+def rel_err(acc, a64):
+    return np.linalg.norm(acc - a64, axis=1) / np.linalg.norm(a64, axis=1)
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 63, 64, 65, 1000, 4097])
+def test_small_trees_equal_direct_sum_when_every_node_opens(oracle, N):
+    rng = np.random.default_rng(N)
+    with lbvh_sim(N, theta=1e-6) as s:
+        s.pos[:] = rng.normal(size=(N, 3)).astype(np.float32)
+        s.mas[:] = rng.uniform(0.5, 1.5, N).astype(np.float32) / N
+        s.solve()
+        ex = s.lbvh_export()
+        assert ex["bad"] == 0
+        assert ex["cells"] == 0 and ex["parts"] == N * N      # all leaf sums
+        a64, p64 = oracle.pp_f64(s.pos.copy(), s.mas.copy(), 1e-2, acc_phys=1)
+        scale = np.linalg.norm(a64, axis=1).max() + 1e-30
+        assert np.abs(s.acc - a64).max() <= 2e-6 * scale
+        assert np.allclose(s.pot, p64, rtol=2e-6)
+    L.set_params()
+
+
+def test_morton_codes_sort_and_tree_invariants(oracle):
+    N = 50000
+    with lbvh_sim(N) as s:
+        s.ic_plummer(seed=4, first=0, count=N)
+        s.pos[:5] = s.pos[5:10]          # duplicate positions: equal codes
+        s.solve()
+        ex = s.lbvh_export(nodes=True)
+        keys = oracle.morton63(s.pos.copy())
+        order = np.argsort(keys, kind="stable")
+        assert np.array_equal(ex["keys"], keys[order])
+        assert np.array_equal(ex["order"], order.astype(np.uint32))
+        # every internal node covers a contiguous range split exactly by its children
+        lr = ex["node_lr_range"]
+        def rng_of(c):
+            return (c - (N - 1), 1) if c >= N - 1 else (lr[c, 2], lr[c, 3])
+        assert tuple(lr[0, 2:]) == (0, N)
+        f_l, n_l = np.array([rng_of(c) for c in lr[:, 0]]).T
+        f_r, n_r = np.array([rng_of(c) for c in lr[:, 1]]).T
+        assert np.array_equal(f_l, lr[:, 2]) and np.array_equal(f_l + n_l, f_r)
+        assert np.array_equal(n_l + n_r, lr[:, 3])
+        # every node id is referenced exactly once as a child
+        kids = np.sort(np.concatenate([lr[:, 0], lr[:, 1]]))
+        assert np.array_equal(kids, np.arange(1, 2 * N - 1))
+        # monopoles: masses and centres of mass of the ranges (FP64 check)
+        cm = ex["node_cm"]
+        m_sorted = s.mas[order].astype(np.float64)
+        p_sorted = s.pos[order].astype(np.float64)
+        cum_m = np.concatenate([[0], np.cumsum(m_sorted)])
+        cum_mp = np.concatenate([np.zeros((1, 3)), np.cumsum(m_sorted[:, None] * p_sorted, 0)])
+        first, count = lr[:, 2], lr[:, 3]
+        M = cum_m[first + count] - cum_m[first]
+        com = (cum_mp[first + count] - cum_mp[first]) / M[:, None]
+        assert np.allclose(cm[:N - 1, 3], M, rtol=1e-5)
+        assert np.all(np.abs(cm[:N - 1, :3] - com) <= 1e-5 * (1.0 + np.abs(com)))
+        assert np.array_equal(bits(cm[N - 1:]), bits(np.column_stack([s.pos[order], s.mas[order]])))
+    L.set_params()
+
+
+@pytest.mark.parametrize("theta", [0.3, 0.5, 0.7])
+def test_force_error_against_fp64_direct_sum(oracle, theta):
+    N = 131072
+    with lbvh_sim(N, theta=theta, epsilon=5e-3, mem_mode=L.SF_MEM_RESIDENT) as s:
+        s.ic_plummer(seed=21, first=0, count=N)
+        s.solve()
+        s.to_host()
+        ex = s.lbvh_export()
+        assert ex["bad"] == 0
+        idx = np.arange(0, N, 32, dtype=np.uint32)
+        a64, p64 = s.direct_f64(idx)
+        e = rel_err(s.acc[idx], a64)
+        print(f"theta {theta}: rel acc error p50 {np.percentile(e, 50):.3g} p90 {np.percentile(e, 90):.3g} "
+              f"p99 {np.percentile(e, 99):.3g} max {e.max():.3g}; interactions/particle "
+              f"{(ex['cells'] + ex['parts']) / N:.0f}")
+        limit = {0.3: 2e-4, 0.5: 1e-3, 0.7: 3e-3}[theta]
+        assert np.percentile(e, 50) <= limit
+        assert np.abs(s.pot[idx] / p64 - 1).max() < 1e-2
+    L.set_params()
+
+
+def test_walk_is_deterministic():
+    N = 20000
+    outs = []
+    for _ in range(2):
+        with lbvh_sim(N) as s:
+            s.ic_plummer(seed=9, first=0, count=N)
+            s.solve()
+            outs.append((s.acc.copy(), s.pot.copy()))
+    assert np.array_equal(bits(outs[0][0]), bits(outs[1][0]))
+    assert np.array_equal(bits(outs[0][1]), bits(outs[1][1]))
+    L.set_params()
+
+
+def test_lbvh_kdk_steps_and_reference_acc_mode(oracle):
+    N = 8192
+    with lbvh_sim(N, theta=0.4, acc_mode=L.SF_ACC_REFERENCE, precision=L.SF_PRECISION_EXACT) as s:
+        s.ic_plummer(seed=1, first=0, count=N)
+        s.solve()
+        a64, _ = oracle.pp_f64(s.pos.copy(), s.mas.copy(), 1e-2, acc_phys=0)
+        assert np.median(rel_err(s.acc, a64)) < 1e-3
+        for n in range(3):
+            s.step()
+        assert np.isfinite(s.pos).all() and np.isfinite(s.acc).all()
+    L.set_params()

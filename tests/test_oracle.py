@@ -251,3 +251,29 @@ def test_cubic_sph_uniform_density_and_momentum(oracle):
     assert np.abs((mas[:, None] * acc.reshape(N, 3)).sum(0)).max() < 1e-12
     dE = (mas[:, None] * vel * acc.reshape(N, 3)).sum() + (mas * du).sum()
     assert abs(dE) < 1e-12
+
+
+def test_morton_codes_match_magic_number_interleave(oracle):
+    # the oracle's bit-by-bit interleave against the spread-by-masks form
+    rng = np.random.default_rng(8)
+    pos = np.concatenate([rng.normal(size=(3000, 3)), [[0, 0, 0], [1e-30, -1e-30, 0]]]).astype(np.float32)
+    key = oracle.morton63(pos)
+
+    def spread(v):
+        v = v.astype(np.uint64) & np.uint64(0x1FFFFF)
+        for s, m in ((32, 0x1F00000000FFFF), (16, 0x1F0000FF0000FF), (8, 0x100F00F00F00F00F),
+                     (4, 0x10C30C30C30C30C3), (2, 0x1249249249249249)):
+            v = (v | (v << np.uint64(s))) & np.uint64(m)
+        return v
+
+    p64 = pos.astype(np.float64)
+    lo = p64.min(0)
+    ext = (p64.max(0) - lo).max() * (1.0 + 1e-9)
+    q = np.clip((p64 - lo) * (2097152.0 / ext), 0.0, 2097151.0).astype(np.uint64)
+    ref = (spread(q[:, 0]) << np.uint64(2)) | (spread(q[:, 1]) << np.uint64(1)) | spread(q[:, 2])
+    assert np.array_equal(key, ref)
+    assert key.max() < 2 ** 63
+    # Morton order preserves octant nesting: the top 3 bits are the octant
+    # of the first bisection
+    oct_ = ((q[:, 0] >> np.uint64(20)) << np.uint64(2)) | ((q[:, 1] >> np.uint64(20)) << np.uint64(1)) | (q[:, 2] >> np.uint64(20))
+    assert np.array_equal(key >> np.uint64(60), oct_)
