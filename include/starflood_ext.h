@@ -165,14 +165,15 @@ int sf_ic_plummer(sim_t* sim_ptr, uint32_t seed, unsigned int first, unsigned in
 
 /* Diagnostics of the SF_TREE_LBVH tree last built for tree_ptr (host
    outputs; any may be NULL): the sorted 63-bit Morton codes and the sorting
-   permutation (N each); counts[3] = cell interactions and particle
-   interactions of the last walk, and the number of wavefront groups that met
-   a malformed tree (0 unless something is broken); node_cm = 2N-1 records
+   permutation (N each); counts[4] = cell interactions and particle
+   interactions of the last walk, the number of wavefronts that met a
+   malformed tree (0 unless something is broken) and the number of 64-node
+   batches the walk classified; node_cm = 2N-1 records
    (com x, y, z, mass), internal nodes first; node_lr_range = N-1 records
    (left, right, first, count), child ids >= N-1 being leaves of sorted
    particle id - (N-1). */
 int sf_lbvh_export(const tree_t* tree_ptr, unsigned long long* keys, unsigned int* order,
-                   unsigned long long counts[3], float* node_cm, int* node_lr_range);
+                   unsigned long long counts[4], float* node_cm, int* node_lr_range);
 
 /* --- kernel timing with HIP events on the library stream --- */
 void sf_prof_enable(int on);

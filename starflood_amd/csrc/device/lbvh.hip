@@ -1,4 +1,5 @@
-// lbvh.hip -- the SF_TREE_LBVH Barnes-Hut tree: build and walk on gfx950.
+// lbvh.hip -- the SF_TREE_LBVH Barnes-Hut tree: build on gfx950 (the walk is
+// in lbvh_walk.hip).
 //
 // Build (every step, O(N), bandwidth-bound):
 //  1. FP64 bounding box (min/max are exact, so any reduction order gives the
@@ -16,16 +17,9 @@
 //     fixed order, so the result does not depend on scheduling (no float
 //     atomics).
 //
-// Walk: one wavefront per group of 64 consecutive sorted particles.  Up to 64
-// pending nodes are classified at once, one per lane, against the group's
-// bounding box: accepted as a monopole when the box stays outside the node's
-// critical radius r = l/theta + |com - centre| (l = longest side of the
-// node's particle box), summed particle by particle when the node holds at
-// most leaf_size particles, opened otherwise.  Accepted cells and leaf ranges
-// are queued in LDS and evaluated 64 at a time by every lane (the same
-// 13-VALU + rsqrt pair interaction as the PP kernel), so the tree logic costs
-// ~1/64 of the interactions.  The self pair is included in leaf sums, as in
-// the reference's PP sum.
+// Walk records (k_records) carry, per node, the monopole, the squared
+// critical radius for the current theta and either the particle range or the
+// two children.
 #include <hip/hip_runtime.h>
 
 #include "../sf_internal.h"
@@ -343,187 +337,6 @@ __global__ __launch_bounds__(kBlock) void k_records(int N, const int2* __restric
 	rec_b[n] = make_int4(__float_as_int(rc * rc), leafy ? first : l, leafy ? count : r, leafy ? count : -count);
 }
 
-// ---------------------------------------------------------------- walk
-constexpr int kStack = 2048;       // node ids per wavefront
-constexpr int kHead = 160;         // room for a depth-first descent
-constexpr int kList = 128;         // queued cells / leaves per wavefront
-constexpr int kWaves = kBlock / 64;
-
-__device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
-
-__device__ __forceinline__ float wave_min(float v) {
-	for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
-	return v;
-}
-__device__ __forceinline__ float wave_max(float v) {
-	for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-	return v;
-}
-
-struct Acc {
-	float tx, ty, tz, tp;   // tile partial sums
-	float sx, sy, sz, sp;   // running sums
-	float cx, cy, cz, cp;   // Kahan compensations
-};
-
-__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
-	const float y = x - c;
-	const float t = s + y;
-	c = (t - s) - y;
-	s = t;
-}
-
-__device__ __forceinline__ void fold(Acc& a) {
-	kahan_add(a.sx, a.cx, a.tx);
-	kahan_add(a.sy, a.cy, a.ty);
-	kahan_add(a.sz, a.cz, a.tz);
-	kahan_add(a.sp, a.cp, a.tp);
-	a.tx = a.ty = a.tz = a.tp = 0.0f;
-}
-
-__device__ __forceinline__ void interact(Acc& a, float xi, float yi, float zi, float4 bj, float eps2) {
-	const float dx = bj.x - xi, dy = bj.y - yi, dz = bj.z - zi;
-	float r2 = __builtin_fmaf(dx, dx, eps2);
-	r2 = __builtin_fmaf(dy, dy, r2);
-	r2 = __builtin_fmaf(dz, dz, r2);
-	const float inv = __builtin_amdgcn_rsqf(r2);
-	const float mi = bj.w * inv;
-	const float w = mi * (inv * inv);
-	a.tx = __builtin_fmaf(w, dx, a.tx);
-	a.ty = __builtin_fmaf(w, dy, a.ty);
-	a.tz = __builtin_fmaf(w, dz, a.tz);
-	a.tp += mi;
-}
-
-__global__ __launch_bounds__(kBlock) void k_walk(const float4* __restrict__ body, const unsigned* __restrict__ order,
-                                                 int N, const float4* __restrict__ rec_a,
-                                                 const int4* __restrict__ rec_b, float eps2, double G, int acc_phys,
-                                                 float* __restrict__ acc, float* __restrict__ pot,
-                                                 unsigned long long* __restrict__ counters) {
-	__shared__ int s_stack[kWaves][kStack];
-	__shared__ float4 s_cell[kWaves][kList];
-	__shared__ int2 s_leaf[kWaves][kList];
-	const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-	const int group = (int)(blockIdx.x * kWaves + w);
-	const int base = group * 64;
-	if (base >= N) return;   // whole wavefront leaves together
-	const int p = base + (int)lane;
-	const bool live = p < N;
-	const float4 me = body[live ? p : N - 1];
-	const float xi = me.x, yi = me.y, zi = me.z;
-	const float gx0 = wave_min(xi), gy0 = wave_min(yi), gz0 = wave_min(zi);
-	const float gx1 = wave_max(xi), gy1 = wave_max(yi), gz1 = wave_max(zi);
-	int* stack = s_stack[w];
-	float4* cells = s_cell[w];
-	int2* leaves = s_leaf[w];
-	const unsigned long long lt = (1ull << lane) - 1ull;
-
-	Acc a;
-	a.tx = a.ty = a.tz = a.tp = a.sx = a.sy = a.sz = a.sp = a.cx = a.cy = a.cz = a.cp = 0.0f;
-	unsigned long long n_cell = 0, n_part = 0;
-	const int n_nodes = 2 * N - 1;
-	const unsigned budget = 2u * (unsigned)n_nodes + 64u;
-	unsigned popped = 0, bad = 0;
-	int sp = 0, nc = 0, nl = 0;
-	if (lane == 0) stack[0] = 0;   // root (internal node 0, or the single leaf when N == 1)
-	sp = 1;
-	wave_fence();
-
-	while (sp > 0 || nc > 0 || nl > 0) {
-		if (sp > 0) {
-			const int k = (sp + 64 <= kStack - kHead) ? min(64, sp) : 1;
-			int node = -1;
-			if ((int)lane < k) node = stack[sp - 1 - (int)lane];
-			sp -= k;
-			wave_fence();
-			popped += (unsigned)k;
-			int cls = 0;   // 0 none, 1 cell, 2 leaf, 3 open
-			float4 ra = make_float4(0.f, 0.f, 0.f, 0.f);
-			int4 rb = make_int4(0, 0, 0, 0);
-			if (node >= n_nodes) {
-				node = -1;   // only a malformed tree gets here
-				bad = 1u;
-			}
-			if (node >= 0) {
-				ra = rec_a[node];
-				rb = rec_b[node];
-				const float dx = fmaxf(fmaxf(gx0 - ra.x, ra.x - gx1), 0.0f);
-				const float dy = fmaxf(fmaxf(gy0 - ra.y, ra.y - gy1), 0.0f);
-				const float dz = fmaxf(fmaxf(gz0 - ra.z, ra.z - gz1), 0.0f);
-				const float d2 = dx * dx + dy * dy + dz * dz;
-				const int count = abs(rb.w);
-				if (d2 > __int_as_float(rb.x) && count > 1) cls = 1;
-				else if (rb.w > 0) cls = 2;   // (first, count) record
-				else cls = 3;
-			}
-			const unsigned long long m_cell = __ballot(cls == 1);
-			const unsigned long long m_leaf = __ballot(cls == 2);
-			const unsigned long long m_open = __ballot(cls == 3);
-			if (cls == 1) cells[nc + __popcll(m_cell & lt)] = ra;
-			if (cls == 2) leaves[nl + __popcll(m_leaf & lt)] = make_int2(rb.y, rb.z);
-			if (cls == 3) {
-				const int at = sp + 2 * __popcll(m_open & lt);
-				if (at + 1 < kStack) {   // always true by the depth argument above
-					stack[at] = rb.y;
-					stack[at + 1] = rb.z;
-				} else {
-					bad = 1u;
-				}
-			}
-			nc += __popcll(m_cell);
-			nl += __popcll(m_leaf);
-			sp = min(sp + 2 * __popcll(m_open), kStack);
-			wave_fence();
-			// a tree visits each node at most once per group
-			if (popped > budget) {
-				bad = 1u;
-				sp = 0;
-			}
-		}
-		const bool drain = sp == 0;
-		if (nc >= 64 || (drain && nc > 0)) {
-			const int take = min(nc, 64);
-			for (int q = 0; q < take; ++q) interact(a, xi, yi, zi, cells[q], eps2);
-			fold(a);
-			n_cell += (unsigned long long)take;
-			wave_fence();
-			if ((int)lane < nc - take) cells[lane] = cells[take + lane];
-			nc -= take;
-			wave_fence();
-		}
-		if (nl >= 64 || (drain && nl > 0)) {
-			const int take = min(nl, 64);
-			for (int q = 0; q < take; ++q) {
-				const int2 lf = leaves[q];
-				for (int j = lf.x; j < lf.x + lf.y; ++j) interact(a, xi, yi, zi, body[j], eps2);
-				n_part += (unsigned long long)lf.y;
-				fold(a);
-			}
-			wave_fence();
-			if ((int)lane < nl - take) leaves[lane] = leaves[take + lane];
-			nl -= take;
-			wave_fence();
-		}
-	}
-	if (live) {
-		const unsigned i = order[p];
-		const double sx = (double)a.sx - (double)a.cx, sy = (double)a.sy - (double)a.cy,
-		             sz = (double)a.sz - (double)a.cz, sp2 = (double)a.sp - (double)a.cp;
-		const double f = acc_phys ? G : G * (double)me.w;
-		acc[3u * i + 0u] = (float)(f * sx);
-		acc[3u * i + 1u] = (float)(f * sy);
-		acc[3u * i + 2u] = (float)(f * sz);
-		pot[i] = (float)(-G * sp2);
-	}
-	const bool any_bad = __ballot(bad != 0u) != 0ull;
-	if (counters && lane == 0) {
-		const unsigned long long nlive = (unsigned long long)min(64, N - base);
-		atomicAdd(&counters[0], n_cell * nlive);
-		atomicAdd(&counters[1], n_part * nlive);
-		if (any_bad) atomicAdd(&counters[2], 1ull);
-	}
-}
-
 }  // namespace
 
 extern "C" {
@@ -596,18 +409,5 @@ int sfd_lbvh_records(unsigned N, const void* node_lr, const void* node_range, co
 	return sf_hip_check(hipGetLastError(), "sfd_lbvh_records", "launch");
 }
 
-int sfd_lbvh_walk(const void* body, const unsigned* order, unsigned N, const void* rec_a, const void* rec_b,
-                  float eps2, double G, int acc_phys, float* acc, float* pot, unsigned long long* counters,
-                  hipStream_t s) {
-	if (N == 0) return 0;
-	const unsigned groups = (N + 63) / 64;
-	if (counters) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, counters, 4u);
-	const int tok = sf_prof_begin("lbvh_walk", s);
-	hipLaunchKernelGGL(k_walk, dim3((groups + kWaves - 1) / kWaves), dim3(kBlock), 0, s,
-	                   static_cast<const float4*>(body), order, (int)N, static_cast<const float4*>(rec_a),
-	                   static_cast<const int4*>(rec_b), eps2, G, acc_phys, acc, pot, counters);
-	sf_prof_end(tok, s);
-	return sf_hip_check(hipGetLastError(), "sfd_lbvh_walk", "launch");
-}
 
 }  // extern "C"

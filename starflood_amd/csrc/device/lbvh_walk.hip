@@ -1,0 +1,278 @@
+// lbvh_walk.hip -- the SF_TREE_LBVH walk: one wavefront per group of 64
+// consecutive sorted particles.
+//
+// Each step pops up to 64 pending nodes, one per lane, and classifies them
+// against the group's bounding box: a monopole "cell" when the box stays
+// outside the node's critical radius r = l/theta + |com - centre| (l = longest
+// side of the node's particle box), a "leaf" summed particle by particle when
+// the node holds at most leaf_size particles, otherwise opened (its two
+// children are pushed).  Cells and leaf particles are queued in LDS and
+// evaluated 64 at a time by every lane with the PP kernel's 13-VALU + rsqrt
+// interaction; leaf particles are gathered 64 at a time with one vector load,
+// so the tree logic and memory latency are amortised over 64 interactions.
+//
+// The grid is persistent: each wavefront takes group after group from an
+// atomic counter, and owns a node stack in global memory (deep enough that
+// the 64-wide pops never have to narrow).  Every wavefront exits when the
+// counter passes the last group.  The self pair is included in leaf sums, as
+// in the reference's PP sum.
+#include <hip/hip_runtime.h>
+
+#include "../sf_internal.h"
+#include "tree_dev.h"
+
+namespace {
+
+constexpr unsigned kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kList = 128;          // queued cells / leaves / particle ids per wavefront
+constexpr int kStack = 8192;        // node ids per wavefront (global memory)
+constexpr int kHead = 160;          // room for a depth-first descent (tree depth <= 96)
+
+__device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+__device__ __forceinline__ float wave_min(float v) {
+	for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+	return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+	for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+	return v;
+}
+
+struct Acc {
+	float tx, ty, tz, tp;   // partial sums of the current batch of 64
+	float sx, sy, sz, sp;   // running sums
+	float cx, cy, cz, cp;   // Kahan compensations
+};
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
+	const float y = x - c;
+	const float t = s + y;
+	c = (t - s) - y;
+	s = t;
+}
+
+__device__ __forceinline__ void fold(Acc& a) {
+	kahan_add(a.sx, a.cx, a.tx);
+	kahan_add(a.sy, a.cy, a.ty);
+	kahan_add(a.sz, a.cz, a.tz);
+	kahan_add(a.sp, a.cp, a.tp);
+	a.tx = a.ty = a.tz = a.tp = 0.0f;
+}
+
+__device__ __forceinline__ void interact(Acc& a, float xi, float yi, float zi, float4 bj, float eps2) {
+	const float dx = bj.x - xi, dy = bj.y - yi, dz = bj.z - zi;
+	float r2 = __builtin_fmaf(dx, dx, eps2);
+	r2 = __builtin_fmaf(dy, dy, r2);
+	r2 = __builtin_fmaf(dz, dz, r2);
+	const float inv = __builtin_amdgcn_rsqf(r2);
+	const float mi = bj.w * inv;
+	const float w = mi * (inv * inv);
+	a.tx = __builtin_fmaf(w, dx, a.tx);
+	a.ty = __builtin_fmaf(w, dy, a.ty);
+	a.tz = __builtin_fmaf(w, dz, a.tz);
+	a.tp += mi;
+}
+
+// Interactions with q[0 .. n), n <= 64, read from LDS by every lane.
+__device__ __forceinline__ void eval_list(Acc& a, float xi, float yi, float zi, const float4* q, int n, float eps2) {
+	if (n == 64) {
+#pragma unroll 8
+		for (int k = 0; k < 64; ++k) interact(a, xi, yi, zi, q[k], eps2);
+	} else {
+		for (int k = 0; k < n; ++k) interact(a, xi, yi, zi, q[k], eps2);
+	}
+	fold(a);
+}
+
+__global__ __launch_bounds__(kBlock) void k_walk(const float4* __restrict__ body, const unsigned* __restrict__ order,
+                                                 int N, const float4* __restrict__ rec_a,
+                                                 const int4* __restrict__ rec_b, float eps2, double G, int acc_phys,
+                                                 float* __restrict__ acc, float* __restrict__ pot,
+                                                 unsigned long long* __restrict__ counters, int* __restrict__ gstack,
+                                                 unsigned* __restrict__ next_group, unsigned n_groups) {
+	__shared__ float4 s_cell[kWaves][kList];
+	__shared__ int2 s_leaf[kWaves][kList];
+	__shared__ int s_pid[kWaves][kList];
+	__shared__ float4 s_part[kWaves][64];
+	const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+	int* stack = gstack + (size_t)(blockIdx.x * kWaves + w) * kStack;
+	float4* cells = s_cell[w];
+	int2* leaves = s_leaf[w];
+	int* pid = s_pid[w];
+	float4* part = s_part[w];
+	const unsigned long long lt = (1ull << lane) - 1ull;
+	const int n_nodes = 2 * N - 1;
+	const unsigned budget = 2u * (unsigned)n_nodes + 64u;
+	unsigned long long n_cell_tot = 0, n_part_tot = 0, n_batch = 0;
+	unsigned bad = 0;
+
+	for (;;) {
+		unsigned g = 0;
+		if (lane == 0) g = atomicAdd(next_group, 1u);
+		g = __shfl(g, 0);
+		if (g >= n_groups) break;
+		const int base = (int)g * 64;
+		const int p = base + (int)lane;
+		const bool live = p < N;
+		const float4 me = body[live ? p : N - 1];
+		const float xi = me.x, yi = me.y, zi = me.z;
+		const float gx0 = wave_min(xi), gy0 = wave_min(yi), gz0 = wave_min(zi);
+		const float gx1 = wave_max(xi), gy1 = wave_max(yi), gz1 = wave_max(zi);
+
+		Acc a;
+		a.tx = a.ty = a.tz = a.tp = a.sx = a.sy = a.sz = a.sp = a.cx = a.cy = a.cz = a.cp = 0.0f;
+		unsigned long long n_cell = 0, n_part = 0;
+		unsigned popped = 0;
+		int sp = 1, nc = 0, nl = 0, npid = 0;
+		if (lane == 0) stack[0] = 0;   // root: internal node 0, or the only leaf when N == 1
+		wave_fence();
+
+		while (sp > 0 || nc > 0 || nl > 0 || npid > 0) {
+			if (sp > 0) {
+				const int k = (sp + 64 <= kStack - kHead) ? min(64, sp) : 1;
+				int node = -1;
+				if ((int)lane < k) node = stack[sp - 1 - (int)lane];
+				sp -= k;
+				popped += (unsigned)k;
+				++n_batch;
+				wave_fence();
+				if (node >= n_nodes) {
+					node = -1;   // only a malformed tree gets here
+					bad = 1u;
+				}
+				int cls = 0;   // 0 none, 1 cell, 2 leaf, 3 open
+				float4 ra = make_float4(0.f, 0.f, 0.f, 0.f);
+				int4 rb = make_int4(0, 0, 0, 0);
+				if (node >= 0) {
+					ra = rec_a[node];
+					rb = rec_b[node];
+					const float dx = fmaxf(fmaxf(gx0 - ra.x, ra.x - gx1), 0.0f);
+					const float dy = fmaxf(fmaxf(gy0 - ra.y, ra.y - gy1), 0.0f);
+					const float dz = fmaxf(fmaxf(gz0 - ra.z, ra.z - gz1), 0.0f);
+					const float d2 = dx * dx + dy * dy + dz * dz;
+					if (d2 > __int_as_float(rb.x) && abs(rb.w) > 1) cls = 1;
+					else if (rb.w > 0) cls = 2;   // (first, count) record
+					else cls = 3;
+				}
+				const unsigned long long m_cell = __ballot(cls == 1);
+				const unsigned long long m_leaf = __ballot(cls == 2);
+				const unsigned long long m_open = __ballot(cls == 3);
+				if (cls == 1) cells[nc + __popcll(m_cell & lt)] = ra;
+				if (cls == 2) leaves[nl + __popcll(m_leaf & lt)] = make_int2(rb.y, rb.z);
+				if (cls == 3) {
+					const int at = sp + 2 * __popcll(m_open & lt);
+					if (at + 1 < kStack) {   // always true: see kHead
+						stack[at] = rb.y;
+						stack[at + 1] = rb.z;
+					} else {
+						bad = 1u;
+					}
+				}
+				nc += __popcll(m_cell);
+				nl += __popcll(m_leaf);
+				sp = min(sp + 2 * __popcll(m_open), kStack);
+				wave_fence();
+				if (popped > budget) {   // a tree visits each node at most once per group
+					bad = 1u;
+					sp = 0;
+				}
+			}
+			const bool drain = sp == 0;
+			// monopoles
+			if (nc >= 64 || (drain && nc > 0)) {
+				const int take = min(nc, 64);
+				eval_list(a, xi, yi, zi, cells, take, eps2);
+				n_cell += (unsigned long long)take;
+				wave_fence();
+				if ((int)lane < nc - take) cells[lane] = cells[take + lane];
+				nc -= take;
+				wave_fence();
+			}
+			// leaves: expand into particle ids, then gather and sum 64 at a time
+			if (nl >= 64 || (drain && (nl > 0 || npid > 0))) {
+				int q = 0;
+				while (q < nl || (drain && npid > 0)) {
+					while (q < nl && npid < 64) {
+						const int2 lf = leaves[q];
+						if ((int)lane < lf.y) pid[npid + (int)lane] = lf.x + (int)lane;
+						npid += lf.y;
+						++q;
+					}
+					wave_fence();
+					if (npid >= 64 || (drain && q == nl && npid > 0)) {
+						const int take = min(npid, 64);
+						if ((int)lane < take) part[lane] = body[pid[lane]];
+						wave_fence();
+						eval_list(a, xi, yi, zi, part, take, eps2);
+						n_part += (unsigned long long)take;
+						wave_fence();
+						if ((int)lane < npid - take) pid[lane] = pid[take + (int)lane];
+						npid -= take;
+						wave_fence();
+					} else {
+						break;
+					}
+				}
+				wave_fence();
+				if ((int)lane < nl - q) leaves[lane] = leaves[q + (int)lane];
+				nl -= q;
+				wave_fence();
+			}
+		}
+		if (live) {
+			const unsigned i = order[p];
+			const double sx = (double)a.sx - (double)a.cx, sy = (double)a.sy - (double)a.cy,
+			             sz = (double)a.sz - (double)a.cz, sp2 = (double)a.sp - (double)a.cp;
+			const double f = acc_phys ? G : G * (double)me.w;
+			acc[3u * i + 0u] = (float)(f * sx);
+			acc[3u * i + 1u] = (float)(f * sy);
+			acc[3u * i + 2u] = (float)(f * sz);
+			pot[i] = (float)(-G * sp2);
+		}
+		const unsigned long long nlive = (unsigned long long)min(64, N - base);
+		n_cell_tot += n_cell * nlive;
+		n_part_tot += n_part * nlive;
+	}
+	const bool any_bad = __ballot(bad != 0u) != 0ull;
+	if (counters && lane == 0) {
+		atomicAdd(&counters[0], n_cell_tot);
+		atomicAdd(&counters[1], n_part_tot);
+		if (any_bad) atomicAdd(&counters[2], 1ull);
+		atomicAdd(&counters[3], n_batch);
+	}
+}
+
+__global__ void k_walk_reset(unsigned long long* __restrict__ counters, unsigned* __restrict__ next_group) {
+	if (threadIdx.x < 4) counters[threadIdx.x] = 0ull;
+	if (threadIdx.x == 0) *next_group = 0u;
+}
+
+}  // namespace
+
+extern "C" {
+
+// persistent grid: enough blocks to fill the chip several times over
+unsigned sfd_lbvh_walk_blocks(void) { return 256u * 8u; }
+
+size_t sfd_lbvh_walk_stack_bytes(void) { return (size_t)sfd_lbvh_walk_blocks() * kWaves * kStack * sizeof(int); }
+
+int sfd_lbvh_walk(const void* body, const unsigned* order, unsigned N, const void* rec_a, const void* rec_b,
+                  float eps2, double G, int acc_phys, float* acc, float* pot, unsigned long long* counters,
+                  int* gstack, unsigned* next_group, hipStream_t s) {
+	if (N == 0) return 0;
+	if (!counters || !gstack || !next_group) return -1;
+	const unsigned groups = (N + 63) / 64;
+	unsigned blocks = (groups + kWaves - 1) / kWaves;
+	if (blocks > sfd_lbvh_walk_blocks()) blocks = sfd_lbvh_walk_blocks();
+	hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, s, counters, next_group);
+	const int tok = sf_prof_begin("lbvh_walk", s);
+	hipLaunchKernelGGL(k_walk, dim3(blocks), dim3(kBlock), 0, s, static_cast<const float4*>(body),
+	                   order, (int)N, static_cast<const float4*>(rec_a), static_cast<const int4*>(rec_b), eps2, G,
+	                   acc_phys, acc, pot, counters, gstack, next_group, groups);
+	sf_prof_end(tok, s);
+	return sf_hip_check(hipGetLastError(), "sfd_lbvh_walk", "launch");
+}
+
+}  // extern "C"

@@ -48,11 +48,13 @@ int sfd_lbvh_topology(const float* pos, const float* mas, const unsigned long lo
 int sfd_lbvh_records(unsigned int N, const void* node_lr, const void* node_range, const void* node_cm,
                      const void* box_lo, const void* box_hi, float theta, int leaf_size, void* rec_a, void* rec_b,
                      hipStream_t s);
-/* counters (4 x u64, zeroed here): cell interactions, particle
-   interactions, groups that met a malformed tree */
+/* lbvh_walk.hip.  counters (4 x u64, zeroed here): cell interactions,
+   particle interactions, wavefronts that met a malformed tree, node batches.
+   gstack: sfd_lbvh_walk_stack_bytes() of scratch; next_group: one counter. */
+size_t sfd_lbvh_walk_stack_bytes(void);
 int sfd_lbvh_walk(const void* body, const unsigned int* order, unsigned int N, const void* rec_a, const void* rec_b,
                   float eps2, double G, int acc_phys, float* acc, float* pot, unsigned long long* counters,
-                  hipStream_t s);
+                  int* gstack, unsigned int* next_group, hipStream_t s);
 
 #ifdef __cplusplus
 }

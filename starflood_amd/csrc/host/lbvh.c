@@ -31,12 +31,14 @@ struct sf_lbvh {
 	void* rec_b;              /* 2N-1 int4 */
 	unsigned long long* counters;   /* 4 */
 	unsigned long long* h_counters; /* pinned */
+	int* gstack;                    /* walk node stacks, one per wavefront slot */
+	unsigned int* next_group;       /* walk work counter */
 };
 
 static void free_buffers(sf_lbvh* t) {
 	void* ptrs[] = {t->key0, t->key1, t->val0, t->val1, t->hist, t->bbox_part, t->box, t->body, t->node_lr,
 	                t->node_range, t->parent, t->arrive, t->node_cm, t->box_lo, t->box_hi, t->rec_a, t->rec_b,
-	                t->counters};
+	                t->counters, t->gstack, t->next_group};
 	for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k)
 		if (ptrs[k]) hipFree(ptrs[k]);
 	if (t->h_counters) hipHostFree(t->h_counters);
@@ -83,6 +85,8 @@ static int ensure(sf_lbvh** st, unsigned int N) {
 	rc |= alloc(&t->rec_a, nn * 16u);
 	rc |= alloc(&t->rec_b, nn * 16u);
 	rc |= alloc((void**)&t->counters, 4u * 8u);
+	rc |= alloc((void**)&t->gstack, sfd_lbvh_walk_stack_bytes());
+	rc |= alloc((void**)&t->next_group, 16u);
 	if (!rc && hipHostMalloc((void**)&t->h_counters, 4u * 8u, hipHostMallocDefault) != hipSuccess) rc = -1;
 	if (rc) {
 		sf_error("tree_build", "device allocation for an LBVH of %u particles failed", N);
@@ -127,11 +131,11 @@ int sf_lbvh_walk(sf_lbvh* t, unsigned int N, const float* pos, const float* mas,
 	                     (int)p->leaf_size, t->rec_a, t->rec_b, s))
 		return -1;
 	return sfd_lbvh_walk(t->body, t->val0, N, t->rec_a, t->rec_b, (float)(p->epsilon * p->epsilon), p->G,
-	                     p->acc_mode == SF_ACC_PHYSICAL, acc, pot, t->counters, s);
+	                     p->acc_mode == SF_ACC_PHYSICAL, acc, pot, t->counters, t->gstack, t->next_group, s);
 }
 
 int sf_lbvh_export_impl(sf_lbvh* t, hipStream_t s, unsigned long long* keys, unsigned int* order,
-                        unsigned long long counts[3], float* node_cm, int* node_lr_range) {
+                        unsigned long long counts[4], float* node_cm, int* node_lr_range) {
 	if (!t || t->N == 0) {
 		sf_error("sf_lbvh_export", "no LBVH has been built");
 		return -1;
@@ -142,7 +146,7 @@ int sf_lbvh_export_impl(sf_lbvh* t, hipStream_t s, unsigned long long* keys, uns
 	if (order) SF_HIP(hipMemcpy(order, t->val0, N * 4u, hipMemcpyDeviceToHost), "sf_lbvh_export");
 	if (counts) {
 		SF_HIP(hipMemcpy(t->h_counters, t->counters, 4u * 8u, hipMemcpyDeviceToHost), "sf_lbvh_export");
-		for (int k = 0; k < 3; ++k) counts[k] = t->h_counters[k];
+		for (int k = 0; k < 4; ++k) counts[k] = t->h_counters[k];
 	}
 	if (node_cm) SF_HIP(hipMemcpy(node_cm, t->node_cm, (2u * N - 1u) * 16u, hipMemcpyDeviceToHost), "sf_lbvh_export");
 	if (node_lr_range && N > 1) {

@@ -77,13 +77,16 @@ int sf_prof_begin(const char* name, hipStream_t s) {
 	const int tok = g_nrec;
 	g_rec[tok].name = name_index(name);
 	if (hipEventRecord(g_rec[tok].e0, s) != hipSuccess) return -1;
+	/* taken now, so brackets may nest; an unfinished pair is flushed as if
+	   it ended at its end event's last recording */
+	g_nrec = tok + 1;
+	if (hipEventRecord(g_rec[tok].e1, s) != hipSuccess) return -1;
 	return tok;
 }
 
 void sf_prof_end(int token, hipStream_t s) {
-	if (token < 0) return;
-	if (hipEventRecord(g_rec[token].e1, s) != hipSuccess) return;
-	g_nrec = token + 1;
+	if (token < 0 || token >= g_nrec) return;
+	hipEventRecord(g_rec[token].e1, s);
 }
 
 int sf_prof_query(const char* name, double* total_ms, unsigned long* count) {

@@ -29,7 +29,7 @@ int sf_lbvh_walk(sf_lbvh* st, unsigned int N, const float* pos, const float* mas
 void sf_lbvh_destroy(sf_lbvh* st);
 /* synchronises s, then copies out what is asked for (NULL: skip) */
 int sf_lbvh_export_impl(sf_lbvh* st, hipStream_t s, unsigned long long* keys, unsigned int* order,
-                        unsigned long long counts[3], float* node_cm, int* node_lr_range);
+                        unsigned long long counts[4], float* node_cm, int* node_lr_range);
 
 /* initcond.c: after a host-side write of sim.mem, refresh the device copy
    in SF_MEM_RESIDENT mode */

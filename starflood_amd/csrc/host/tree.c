@@ -231,7 +231,7 @@ int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number) {
 }
 
 int sf_lbvh_export(const tree_t* tree_ptr, unsigned long long* keys, unsigned int* order,
-                   unsigned long long counts[3], float* node_cm, int* node_lr_range) {
+                   unsigned long long counts[4], float* node_cm, int* node_lr_range) {
 	sf_tree_dev* d = tree_dev_lookup(tree_ptr, "sf_lbvh_export");
 	if (!d) return STARFLOOD_FAILURE;
 	SF_HIP(hipDeviceSynchronize(), "sf_lbvh_export");
@@ -242,10 +242,10 @@ int sf_lbvh_export(const tree_t* tree_ptr, unsigned long long* keys, unsigned in
 /* After an LBVH walk on stream s: fail if any wavefront met a malformed
    tree (synchronises s). */
 static int lbvh_check(sf_tree_dev* d, hipStream_t s) {
-	unsigned long long counts[3] = {0, 0, 0};
+	unsigned long long counts[4] = {0, 0, 0, 0};
 	if (!d->lbvh || sf_lbvh_export_impl(d->lbvh, s, NULL, NULL, counts, NULL, NULL)) return -1;
 	if (counts[2]) {
-		sf_error("solve_gravity_part_tree", "%llu wavefront groups met a malformed tree", counts[2]);
+		sf_error("solve_gravity_part_tree", "%llu wavefronts met a malformed tree", counts[2]);
 		return -1;
 	}
 	return 0;
