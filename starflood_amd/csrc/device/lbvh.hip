@@ -12,14 +12,15 @@
 //  4. Binary radix tree from the sorted codes (Karras 2012): internal node i
 //     covers a contiguous range of the sorted particles; equal codes are
 //     separated by their sorted position.
-//  5. Monopole aggregation bottom-up with one integer arrival counter per
-//     internal node; the second child to arrive combines left and right in a
-//     fixed order, so the result does not depend on scheduling (no float
-//     atomics).
+//  5. Monopoles from FP64 prefix sums of (m, m x, m y, m z) over the sorted
+//     particles: a node's mass and moments are differences of two prefix
+//     entries, so no bottom-up pass, atomics or cross-XCD fences are needed,
+//     and the fixed-shape scan makes the result deterministic.  A node's size
+//     is its Morton cell: the cell fixed by the common prefix of its first
+//     and last codes.
 //
-// Walk records (k_records) carry, per node, the monopole, the squared
-// critical radius for the current theta and either the particle range or the
-// two children.
+// Walk records (k_records) carry, per node, the squared critical radius for
+// the current theta and either the particle range or the two children.
 #include <hip/hip_runtime.h>
 
 #include "../sf_internal.h"
@@ -225,8 +226,7 @@ __device__ __forceinline__ int delta(const unsigned long long* __restrict__ key,
 // Internal nodes 0..N-2; node id >= N-1 is the leaf of sorted particle
 // id - (N-1).  node_lr holds (left, right), node_range (first, count).
 __global__ __launch_bounds__(kBlock) void k_karras(const unsigned long long* __restrict__ key, int N,
-                                                   int2* __restrict__ node_lr, int2* __restrict__ node_range,
-                                                   int* __restrict__ parent) {
+                                                   int2* __restrict__ node_lr, int2* __restrict__ node_range) {
 	const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
 	if (i >= N - 1) return;
 	const int d = (delta(key, N, i, i + 1) - delta(key, N, i, i - 1)) >= 0 ? 1 : -1;
@@ -250,70 +250,144 @@ __global__ __launch_bounds__(kBlock) void k_karras(const unsigned long long* __r
 	const int right = (hi == gamma + 1) ? (N - 1) + gamma + 1 : gamma + 1;
 	node_lr[i] = make_int2(left, right);
 	node_range[i] = make_int2(lo, hi - lo + 1);
-	parent[left] = i;
-	parent[right] = i;
-	if (i == 0) parent[0] = -1;
 }
 
-// node_cm: (com x, y, z, mass); node_box: (lo x, y, z, hi x) (hi y, hi z in node_box2)
-__global__ __launch_bounds__(kBlock) void k_aggregate(const float4* __restrict__ body, int N,
-                                                      const int2* __restrict__ node_lr, const int* __restrict__ parent,
-                                                      unsigned* __restrict__ arrive, float4* __restrict__ node_cm,
-                                                      float4* __restrict__ box_lo, float4* __restrict__ box_hi) {
-	const int p = (int)(blockIdx.x * kBlock + threadIdx.x);
-	if (p >= N) return;
-	const int leaf = (N - 1) + p;
-	const float4 b = body[p];
-	node_cm[leaf] = b;
-	box_lo[leaf] = make_float4(b.x, b.y, b.z, 0.0f);
-	box_hi[leaf] = make_float4(b.x, b.y, b.z, 0.0f);
-	__threadfence();
-	int n = parent[leaf];
-	// a radix tree over 63-bit codes plus 32-bit positions is at most 96
-	// levels deep; the step cap only guards against a malformed tree
-	for (int step = 0; n >= 0 && n < N - 1 && step < 128; ++step) {
-		const unsigned old = atomicAdd(&arrive[n], 1u);
-		if (old != 1u) return;        // first arrival: the other child is not done yet
-		__threadfence();
-		const int2 lr = node_lr[n];
-		// the agent-scope fences around the arrival counter make the sibling's
-		// writes (possibly from another XCD) visible to these loads
-		const float4 a = node_cm[lr.x];
-		const float4 c = node_cm[lr.y];
-		const float4 alo = box_lo[lr.x];
-		const float4 clo = box_lo[lr.y];
-		const float4 ahi = box_hi[lr.x];
-		const float4 chi = box_hi[lr.y];
-		const float m = a.w + c.w;
-		float4 cm;
-		if (m > 0.0f) {
-			const float inv = 1.0f / m;
-			cm = make_float4((a.w * a.x + c.w * c.x) * inv, (a.w * a.y + c.w * c.y) * inv,
-			                 (a.w * a.z + c.w * c.z) * inv, m);
-		} else {
-			cm = make_float4(0.5f * (a.x + c.x), 0.5f * (a.y + c.y), 0.5f * (a.z + c.z), 0.0f);
-		}
-		node_cm[n] = cm;
-		box_lo[n] = make_float4(fminf(alo.x, clo.x), fminf(alo.y, clo.y), fminf(alo.z, clo.z), 0.0f);
-		box_hi[n] = make_float4(fmaxf(ahi.x, chi.x), fmaxf(ahi.y, chi.y), fmaxf(ahi.z, chi.z), 0.0f);
-		__threadfence();
-		n = parent[n];
+// ---- FP64 inclusive prefix sums of (m, m x, m y, m z) in sorted order.
+// Fixed shape: 16 consecutive elements per thread, a Hillis-Steele scan of
+// the 256 thread sums per block, one block scanning the block sums.
+struct d4 {
+	double m, x, y, z;
+};
+__device__ __forceinline__ d4 d4_add(d4 a, d4 b) { return {a.m + b.m, a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ d4 moment(float4 b) {
+	const double m = b.w;
+	return {m, m * (double)b.x, m * (double)b.y, m * (double)b.z};
+}
+
+constexpr unsigned kScanPer = 16;
+constexpr unsigned kScanTile = kBlock * kScanPer;
+
+__device__ d4 block_exclusive_scan(d4 v, d4* sh) {
+	const unsigned t = threadIdx.x;
+	sh[t] = v;
+	__syncthreads();
+	for (unsigned off = 1; off < kBlock; off <<= 1) {
+		const d4 o = t >= off ? sh[t - off] : d4{0.0, 0.0, 0.0, 0.0};
+		__syncthreads();
+		sh[t] = d4_add(sh[t], o);
+		__syncthreads();
+	}
+	const d4 excl = t ? sh[t - 1] : d4{0.0, 0.0, 0.0, 0.0};
+	__syncthreads();
+	return excl;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const float4* __restrict__ body, unsigned N,
+                                                       d4* __restrict__ tile_sum) {
+	__shared__ d4 sh[kBlock];
+	const unsigned b0 = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+	d4 s = {0.0, 0.0, 0.0, 0.0};
+	for (unsigned k = 0; k < kScanPer; ++k)
+		if (b0 + k < N) s = d4_add(s, moment(body[b0 + k]));
+	const d4 excl = block_exclusive_scan(s, sh);
+	if (threadIdx.x == kBlock - 1) tile_sum[blockIdx.x] = d4_add(excl, s);
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_tile_sums(d4* __restrict__ tile_sum, unsigned ntiles) {
+	__shared__ d4 sh[kBlock];
+	const unsigned chunk = (ntiles + kBlock - 1) / kBlock;
+	const unsigned b = threadIdx.x * chunk, e = min(b + chunk, ntiles);
+	d4 s = {0.0, 0.0, 0.0, 0.0};
+	for (unsigned k = b; k < e; ++k) s = d4_add(s, tile_sum[k]);
+	d4 run = block_exclusive_scan(s, sh);
+	for (unsigned k = b; k < e; ++k) {
+		const d4 v = tile_sum[k];
+		tile_sum[k] = run;       // exclusive offset of tile k
+		run = d4_add(run, v);
 	}
 }
 
-// Walk records: rec_a = (com, mass), rec_b = (r_crit^2, a, b, count) where
-// (a, b) = (first, count) for nodes of at most leaf_size particles and
-// (left, right) otherwise; leaves of one particle carry count 1.
+__global__ __launch_bounds__(kBlock) void k_scan_write(const float4* __restrict__ body, unsigned N,
+                                                       const d4* __restrict__ tile_off, d4* __restrict__ prefix) {
+	__shared__ d4 sh[kBlock];
+	const unsigned b0 = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+	d4 s = {0.0, 0.0, 0.0, 0.0};
+	for (unsigned k = 0; k < kScanPer; ++k)
+		if (b0 + k < N) s = d4_add(s, moment(body[b0 + k]));
+	d4 run = d4_add(tile_off[blockIdx.x], block_exclusive_scan(s, sh));
+	for (unsigned k = 0; k < kScanPer; ++k) {
+		if (b0 + k < N) {
+			run = d4_add(run, moment(body[b0 + k]));
+			prefix[b0 + k] = run;   // inclusive
+		}
+	}
+}
+
+__device__ __forceinline__ unsigned compact21(unsigned long long v) {
+	v &= 0x1249249249249249ull;
+	v = (v | (v >> 2)) & 0x10C30C30C30C30C3ull;
+	v = (v | (v >> 4)) & 0x100F00F00F00F00Full;
+	v = (v | (v >> 8)) & 0x1F0000FF0000FFull;
+	v = (v | (v >> 16)) & 0x1F00000000FFFFull;
+	v = (v | (v >> 32)) & 0x1FFFFFull;
+	return (unsigned)v;
+}
+
+// Monopole (node_cm) and Morton cell (centre, longest side) of every node.
+__global__ __launch_bounds__(kBlock) void k_node_moments(const float4* __restrict__ body, int N,
+                                                         const unsigned long long* __restrict__ key,
+                                                         const int2* __restrict__ node_range,
+                                                         const d4* __restrict__ prefix, const double* __restrict__ box,
+                                                         float4* __restrict__ node_cm, float4* __restrict__ node_cell) {
+	const int n = (int)(blockIdx.x * kBlock + threadIdx.x);
+	if (n >= 2 * N - 1) return;
+	if (n >= N - 1) {
+		const float4 b = body[n - (N - 1)];
+		node_cm[n] = b;
+		node_cell[n] = make_float4(b.x, b.y, b.z, 0.0f);
+		return;
+	}
+	const int2 rg = node_range[n];
+	const int f = rg.x, l = rg.x + rg.y - 1;
+	const d4 hi = prefix[l];
+	const d4 lo = f > 0 ? prefix[f - 1] : d4{0.0, 0.0, 0.0, 0.0};
+	const double M = hi.m - lo.m;
+	// cell of the common code prefix: p leading bits of the 63-bit code
+	const unsigned long long ka = key[f], kb = key[l];
+	const int p = ka == kb ? 63 : __clzll((long long)(ka ^ kb)) - 1;
+	const double inv_scale = 1.0 / box[3];
+	double centre[3];
+	int bmin = 21;
+	for (int a = 0; a < 3; ++a) {
+		const int bits = max(p - a + 2, 0) / 3;   // code bits fixed on this axis
+		bmin = min(bmin, bits);
+		const unsigned q = compact21(ka >> (2 - a));
+		const unsigned span = 1u << (21 - bits);
+		const unsigned q0 = bits == 0 ? 0u : (q & ~(span - 1u));
+		centre[a] = box[a] + ((double)q0 + 0.5 * (double)span) * inv_scale;
+	}
+	const double side = (double)(1u << (21 - bmin)) * inv_scale;
+	float4 cm;
+	if (M > 0.0) {
+		cm = make_float4((float)((hi.x - lo.x) / M), (float)((hi.y - lo.y) / M), (float)((hi.z - lo.z) / M),
+		                 (float)M);
+	} else {
+		cm = make_float4((float)centre[0], (float)centre[1], (float)centre[2], 0.0f);
+	}
+	node_cm[n] = cm;
+	node_cell[n] = make_float4((float)centre[0], (float)centre[1], (float)centre[2], (float)side);
+}
+
+// Walk records: (r_crit^2, a, b, +-count) with r_crit = side / theta +
+// |com - cell centre|; (a, b) = (first, count) and +count for nodes of at
+// most leaf_size particles, (left, right) and -count otherwise.
 __global__ __launch_bounds__(kBlock) void k_records(int N, const int2* __restrict__ node_lr,
                                                     const int2* __restrict__ node_range,
                                                     const float4* __restrict__ node_cm,
-                                                    const float4* __restrict__ box_lo, const float4* __restrict__ box_hi,
-                                                    float inv_theta, int leaf_size, float4* __restrict__ rec_a,
-                                                    int4* __restrict__ rec_b) {
+                                                    const float4* __restrict__ node_cell, float inv_theta,
+                                                    int leaf_size, int4* __restrict__ rec_b) {
 	const int n = (int)(blockIdx.x * kBlock + threadIdx.x);
 	if (n >= 2 * N - 1) return;
-	const float4 cm = node_cm[n];
-	rec_a[n] = cm;
 	int first, count, l = -1, r = -1;
 	if (n < N - 1) {
 		const int2 rg = node_range[n];
@@ -325,15 +399,10 @@ __global__ __launch_bounds__(kBlock) void k_records(int N, const int2* __restric
 		first = n - (N - 1);
 		count = 1;
 	}
-	const float4 lo = box_lo[n], hi = box_hi[n];
-	const float sx = hi.x - lo.x, sy = hi.y - lo.y, sz = hi.z - lo.z;
-	const float side = fmaxf(sx, fmaxf(sy, sz));
-	const float ox = cm.x - 0.5f * (lo.x + hi.x), oy = cm.y - 0.5f * (lo.y + hi.y), oz = cm.z - 0.5f * (lo.z + hi.z);
-	const float off = sqrtf(ox * ox + oy * oy + oz * oz);
-	const float rc = side * inv_theta + off;
+	const float4 cm = node_cm[n], cell = node_cell[n];
+	const float ox = cm.x - cell.x, oy = cm.y - cell.y, oz = cm.z - cell.z;
+	const float rc = cell.w * inv_theta + sqrtf(ox * ox + oy * oy + oz * oz);
 	const bool leafy = count <= leaf_size;
-	// w carries the particle count, negated for nodes that are opened into
-	// their children (whose ids are then in y, z)
 	rec_b[n] = make_int4(__float_as_int(rc * rc), leafy ? first : l, leafy ? count : r, leafy ? count : -count);
 }
 
@@ -356,6 +425,8 @@ int sfd_lbvh_keys(const float* pos, unsigned N, unsigned n_sort, double* bbox_pa
 
 unsigned sfd_radix_tile(void) { return kTile; }
 
+unsigned sfd_lbvh_scan_tiles(unsigned N) { return (N + kScanTile - 1) / kScanTile; }
+
 // Sorts n_sort (a multiple of the tile) pairs; the result ends in (key0,
 // val0) after the eight passes.
 int sfd_radix_sort(unsigned long long* key0, unsigned* val0, unsigned long long* key1, unsigned* val1,
@@ -374,40 +445,40 @@ int sfd_radix_sort(unsigned long long* key0, unsigned* val0, unsigned long long*
 	return sf_hip_check(hipGetLastError(), "sfd_radix_sort", "launch");
 }
 
+// prefix: N records of 4 doubles; tile_sum: sfd_lbvh_scan_tiles(N) records
 int sfd_lbvh_topology(const float* pos, const float* mas, const unsigned long long* key, const unsigned* order,
-                      unsigned N, void* body, void* node_lr, void* node_range, int* parent, unsigned* arrive,
-                      void* node_cm, void* box_lo, void* box_hi, hipStream_t s) {
+                      unsigned N, const double* box, void* body, void* node_lr, void* node_range, void* prefix,
+                      void* tile_sum, void* node_cm, void* node_cell, hipStream_t s) {
 	const unsigned g = (N + kBlock - 1) / kBlock;
 	float4* b4 = static_cast<float4*>(body);
-	int2* lr = static_cast<int2*>(node_lr);
 	int2* rg = static_cast<int2*>(node_range);
+	d4* pre = static_cast<d4*>(prefix);
+	d4* ts = static_cast<d4*>(tile_sum);
+	const unsigned ntiles = sfd_lbvh_scan_tiles(N);
 	const int tok = sf_prof_begin("lbvh_topology", s);
 	hipLaunchKernelGGL(k_gather_sorted, dim3(g), dim3(kBlock), 0, s, pos, mas, order, N, b4);
-	if (N > 1) {
-		hipLaunchKernelGGL(k_karras, dim3((N - 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, key, (int)N, lr, rg,
-		                   parent);
-		hipLaunchKernelGGL(k_fill_u32, dim3((N - 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, arrive, N - 1, 0u);
-	} else {
-		hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, s, reinterpret_cast<unsigned*>(parent), 1u,
-		                   0xFFFFFFFFu);
-	}
-	hipLaunchKernelGGL(k_aggregate, dim3(g), dim3(kBlock), 0, s, b4, (int)N, lr, parent, arrive,
-	                   static_cast<float4*>(node_cm), static_cast<float4*>(box_lo), static_cast<float4*>(box_hi));
+	if (N > 1)
+		hipLaunchKernelGGL(k_karras, dim3((N - 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, key, (int)N,
+		                   static_cast<int2*>(node_lr), rg);
+	hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s, b4, N, ts);
+	hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kBlock), 0, s, ts, ntiles);
+	hipLaunchKernelGGL(k_scan_write, dim3(ntiles), dim3(kBlock), 0, s, b4, N, ts, pre);
+	const unsigned nn = 2u * N - 1u;
+	hipLaunchKernelGGL(k_node_moments, dim3((nn + kBlock - 1) / kBlock), dim3(kBlock), 0, s, b4, (int)N, key, rg, pre,
+	                   box, static_cast<float4*>(node_cm), static_cast<float4*>(node_cell));
 	sf_prof_end(tok, s);
 	return sf_hip_check(hipGetLastError(), "sfd_lbvh_topology", "launch");
 }
 
-int sfd_lbvh_records(unsigned N, const void* node_lr, const void* node_range, const void* node_cm, const void* box_lo,
-                     const void* box_hi, float theta, int leaf_size, void* rec_a, void* rec_b, hipStream_t s) {
+int sfd_lbvh_records(unsigned N, const void* node_lr, const void* node_range, const void* node_cm,
+                     const void* node_cell, float theta, int leaf_size, void* rec_b, hipStream_t s) {
 	const unsigned nn = 2u * N - 1u;
 	const float inv_theta = theta > 0.0f ? 1.0f / theta : 3.0e38f;
 	hipLaunchKernelGGL(k_records, dim3((nn + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (int)N,
 	                   static_cast<const int2*>(node_lr), static_cast<const int2*>(node_range),
-	                   static_cast<const float4*>(node_cm), static_cast<const float4*>(box_lo),
-	                   static_cast<const float4*>(box_hi), inv_theta, leaf_size, static_cast<float4*>(rec_a),
-	                   static_cast<int4*>(rec_b));
+	                   static_cast<const float4*>(node_cm), static_cast<const float4*>(node_cell), inv_theta,
+	                   leaf_size, static_cast<int4*>(rec_b));
 	return sf_hip_check(hipGetLastError(), "sfd_lbvh_records", "launch");
 }
-
 
 }  // extern "C"

@@ -11,11 +11,10 @@
 // interaction; leaf particles are gathered 64 at a time with one vector load,
 // so the tree logic and memory latency are amortised over 64 interactions.
 //
-// The grid is persistent: each wavefront takes group after group from an
-// atomic counter, and owns a node stack in global memory (deep enough that
-// the 64-wide pops never have to narrow).  Every wavefront exits when the
-// counter passes the last group.  The self pair is included in leaf sums, as
-// in the reference's PP sum.
+// The grid is persistent and sized to the wavefronts resident at once; each
+// wavefront takes every (grid size)-th group and owns a node stack in global
+// memory (deep enough that the 64-wide pops never have to narrow).  The self
+// pair is included in leaf sums, as in the reference's PP sum.
 #include <hip/hip_runtime.h>
 
 #include "../sf_internal.h"
@@ -91,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(const float4* __restrict__ body
                                                  const int4* __restrict__ rec_b, float eps2, double G, int acc_phys,
                                                  float* __restrict__ acc, float* __restrict__ pot,
                                                  unsigned long long* __restrict__ counters, int* __restrict__ gstack,
-                                                 unsigned* __restrict__ next_group, unsigned n_groups) {
+                                                 unsigned n_groups) {
 	__shared__ float4 s_cell[kWaves][kList];
 	__shared__ int2 s_leaf[kWaves][kList];
 	__shared__ int s_pid[kWaves][kList];
@@ -108,11 +107,10 @@ __global__ __launch_bounds__(kBlock) void k_walk(const float4* __restrict__ body
 	unsigned long long n_cell_tot = 0, n_part_tot = 0, n_batch = 0;
 	unsigned bad = 0;
 
-	for (;;) {
-		unsigned g = 0;
-		if (lane == 0) g = atomicAdd(next_group, 1u);
-		g = __shfl(g, 0);
-		if (g >= n_groups) break;
+	// static round-robin over groups: a single shared work counter would make
+	// every wavefront wait on device-scope atomics that serialise across XCDs
+	const unsigned n_slots = gridDim.x * kWaves;
+	for (unsigned g = blockIdx.x * kWaves + w; g < n_groups; g += n_slots) {
 		const int base = (int)g * 64;
 		const int p = base + (int)lane;
 		const bool live = p < N;
@@ -244,33 +242,47 @@ __global__ __launch_bounds__(kBlock) void k_walk(const float4* __restrict__ body
 	}
 }
 
-__global__ void k_walk_reset(unsigned long long* __restrict__ counters, unsigned* __restrict__ next_group) {
+__global__ void k_walk_reset(unsigned long long* __restrict__ counters) {
 	if (threadIdx.x < 4) counters[threadIdx.x] = 0ull;
-	if (threadIdx.x == 0) *next_group = 0u;
+}
+
+// Blocks that are resident at once: every wavefront of the grid runs from the
+// start, so the static group split is balanced (cached after the first call).
+unsigned resident_blocks(void) {
+	static unsigned cached = 0;
+	if (cached) return cached;
+	int per_cu = 0, cus = 0, dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess ||
+	    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_walk, kBlock, 0) != hipSuccess ||
+	    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || per_cu < 1 ||
+	    cus < 1) {
+		(void)hipGetLastError();
+		per_cu = 4;
+		cus = 256;
+	}
+	cached = (unsigned)per_cu * (unsigned)cus;
+	return cached;
 }
 
 }  // namespace
 
 extern "C" {
 
-// persistent grid: enough blocks to fill the chip several times over
-unsigned sfd_lbvh_walk_blocks(void) { return 256u * 8u; }
+size_t sfd_lbvh_walk_stack_bytes(void) { return (size_t)resident_blocks() * kWaves * kStack * sizeof(int); }
 
-size_t sfd_lbvh_walk_stack_bytes(void) { return (size_t)sfd_lbvh_walk_blocks() * kWaves * kStack * sizeof(int); }
-
-int sfd_lbvh_walk(const void* body, const unsigned* order, unsigned N, const void* rec_a, const void* rec_b,
+int sfd_lbvh_walk(const void* body, const unsigned* order, unsigned N, const void* node_cm, const void* rec_b,
                   float eps2, double G, int acc_phys, float* acc, float* pot, unsigned long long* counters,
-                  int* gstack, unsigned* next_group, hipStream_t s) {
+                  int* gstack, hipStream_t s) {
 	if (N == 0) return 0;
-	if (!counters || !gstack || !next_group) return -1;
+	if (!counters || !gstack) return -1;
 	const unsigned groups = (N + 63) / 64;
 	unsigned blocks = (groups + kWaves - 1) / kWaves;
-	if (blocks > sfd_lbvh_walk_blocks()) blocks = sfd_lbvh_walk_blocks();
-	hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, s, counters, next_group);
+	if (blocks > resident_blocks()) blocks = resident_blocks();   // gstack holds resident_blocks() slots
+	hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, s, counters);
 	const int tok = sf_prof_begin("lbvh_walk", s);
-	hipLaunchKernelGGL(k_walk, dim3(blocks), dim3(kBlock), 0, s, static_cast<const float4*>(body),
-	                   order, (int)N, static_cast<const float4*>(rec_a), static_cast<const int4*>(rec_b), eps2, G,
-	                   acc_phys, acc, pot, counters, gstack, next_group, groups);
+	hipLaunchKernelGGL(k_walk, dim3(blocks), dim3(kBlock), 0, s, static_cast<const float4*>(body), order, (int)N,
+	                   static_cast<const float4*>(node_cm), static_cast<const int4*>(rec_b), eps2, G, acc_phys, acc,
+	                   pot, counters, gstack, groups);
 	sf_prof_end(tok, s);
 	return sf_hip_check(hipGetLastError(), "sfd_lbvh_walk", "launch");
 }

@@ -42,19 +42,22 @@ int sfd_lbvh_keys(const float* pos, unsigned int N, unsigned int n_sort, double*
 /* stable 8-pass LSD radix sort; the result is in (key0, val0) */
 int sfd_radix_sort(unsigned long long* key0, unsigned int* val0, unsigned long long* key1, unsigned int* val1,
                    unsigned int n_sort, unsigned int* hist, hipStream_t s);
+/* tiles of the FP64 moment scan (records of 4 doubles each) */
+unsigned int sfd_lbvh_scan_tiles(unsigned int N);
+/* sorted bodies, radix tree, FP64 prefix moments (N records of 4 doubles),
+   node monopoles (2N-1 float4) and Morton cells (2N-1 float4: centre, side) */
 int sfd_lbvh_topology(const float* pos, const float* mas, const unsigned long long* key, const unsigned int* order,
-                      unsigned int N, void* body, void* node_lr, void* node_range, int* parent, unsigned int* arrive,
-                      void* node_cm, void* box_lo, void* box_hi, hipStream_t s);
+                      unsigned int N, const double* box, void* body, void* node_lr, void* node_range, void* prefix,
+                      void* tile_sum, void* node_cm, void* node_cell, hipStream_t s);
 int sfd_lbvh_records(unsigned int N, const void* node_lr, const void* node_range, const void* node_cm,
-                     const void* box_lo, const void* box_hi, float theta, int leaf_size, void* rec_a, void* rec_b,
-                     hipStream_t s);
+                     const void* node_cell, float theta, int leaf_size, void* rec_b, hipStream_t s);
 /* lbvh_walk.hip.  counters (4 x u64, zeroed here): cell interactions,
    particle interactions, wavefronts that met a malformed tree, node batches.
-   gstack: sfd_lbvh_walk_stack_bytes() of scratch; next_group: one counter. */
+   gstack: sfd_lbvh_walk_stack_bytes() of scratch. */
 size_t sfd_lbvh_walk_stack_bytes(void);
-int sfd_lbvh_walk(const void* body, const unsigned int* order, unsigned int N, const void* rec_a, const void* rec_b,
+int sfd_lbvh_walk(const void* body, const unsigned int* order, unsigned int N, const void* node_cm, const void* rec_b,
                   float eps2, double G, int acc_phys, float* acc, float* pot, unsigned long long* counters,
-                  int* gstack, unsigned int* next_group, hipStream_t s);
+                  int* gstack, hipStream_t s);
 
 #ifdef __cplusplus
 }

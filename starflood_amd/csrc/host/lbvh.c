@@ -22,23 +22,20 @@ struct sf_lbvh {
 	void* body;               /* N float4, sorted (x, y, z, m) */
 	void* node_lr;            /* N-1 int2 */
 	void* node_range;         /* N-1 int2 */
-	int* parent;              /* 2N-1 */
-	unsigned int* arrive;     /* N-1 */
-	void* node_cm;            /* 2N-1 float4 */
-	void* box_lo;             /* 2N-1 float4 */
-	void* box_hi;             /* 2N-1 float4 */
-	void* rec_a;              /* 2N-1 float4 */
+	void* prefix;             /* N x 4 doubles: inclusive (m, mx, my, mz) sums */
+	void* tile_sum;           /* scan tiles x 4 doubles */
+	void* node_cm;            /* 2N-1 float4: com, mass */
+	void* node_cell;          /* 2N-1 float4: Morton cell centre, side */
 	void* rec_b;              /* 2N-1 int4 */
 	unsigned long long* counters;   /* 4 */
 	unsigned long long* h_counters; /* pinned */
 	int* gstack;                    /* walk node stacks, one per wavefront slot */
-	unsigned int* next_group;       /* walk work counter */
 };
 
 static void free_buffers(sf_lbvh* t) {
 	void* ptrs[] = {t->key0, t->key1, t->val0, t->val1, t->hist, t->bbox_part, t->box, t->body, t->node_lr,
-	                t->node_range, t->parent, t->arrive, t->node_cm, t->box_lo, t->box_hi, t->rec_a, t->rec_b,
-	                t->counters, t->gstack, t->next_group};
+	                t->node_range, t->prefix, t->tile_sum, t->node_cm, t->node_cell, t->rec_b, t->counters,
+	                t->gstack};
 	for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k)
 		if (ptrs[k]) hipFree(ptrs[k]);
 	if (t->h_counters) hipHostFree(t->h_counters);
@@ -77,16 +74,13 @@ static int ensure(sf_lbvh** st, unsigned int N) {
 	rc |= alloc(&t->body, (size_t)N * 16u);
 	rc |= alloc(&t->node_lr, (size_t)N * 8u);
 	rc |= alloc(&t->node_range, (size_t)N * 8u);
-	rc |= alloc((void**)&t->parent, nn * 4u);
-	rc |= alloc((void**)&t->arrive, (size_t)N * 4u);
+	rc |= alloc(&t->prefix, (size_t)N * 32u);
+	rc |= alloc(&t->tile_sum, (size_t)sfd_lbvh_scan_tiles(N) * 32u);
 	rc |= alloc(&t->node_cm, nn * 16u);
-	rc |= alloc(&t->box_lo, nn * 16u);
-	rc |= alloc(&t->box_hi, nn * 16u);
-	rc |= alloc(&t->rec_a, nn * 16u);
+	rc |= alloc(&t->node_cell, nn * 16u);
 	rc |= alloc(&t->rec_b, nn * 16u);
 	rc |= alloc((void**)&t->counters, 4u * 8u);
 	rc |= alloc((void**)&t->gstack, sfd_lbvh_walk_stack_bytes());
-	rc |= alloc((void**)&t->next_group, 16u);
 	if (!rc && hipHostMalloc((void**)&t->h_counters, 4u * 8u, hipHostMallocDefault) != hipSuccess) rc = -1;
 	if (rc) {
 		sf_error("tree_build", "device allocation for an LBVH of %u particles failed", N);
@@ -111,8 +105,8 @@ int sf_lbvh_build(sf_lbvh** st, unsigned int N, const float* pos, const float* m
 	const int tok = sf_prof_begin("lbvh_build", s);
 	if (sfd_lbvh_keys(pos, N, t->n_sort, t->bbox_part, t->box, t->key0, t->val0, s)) return -1;
 	if (sfd_radix_sort(t->key0, t->val0, t->key1, t->val1, t->n_sort, t->hist, s)) return -1;
-	if (sfd_lbvh_topology(pos, mas, t->key0, t->val0, N, t->body, t->node_lr, t->node_range, t->parent, t->arrive,
-	                      t->node_cm, t->box_lo, t->box_hi, s))
+	if (sfd_lbvh_topology(pos, mas, t->key0, t->val0, N, t->box, t->body, t->node_lr, t->node_range, t->prefix,
+	                      t->tile_sum, t->node_cm, t->node_cell, s))
 		return -1;
 	sf_prof_end(tok, s);
 	return 0;
@@ -127,11 +121,11 @@ int sf_lbvh_walk(sf_lbvh* t, unsigned int N, const float* pos, const float* mas,
 		sf_error("solve_gravity_part_tree", "the tree was built for %u particles, not %u", t ? t->N : 0u, N);
 		return -1;
 	}
-	if (sfd_lbvh_records(N, t->node_lr, t->node_range, t->node_cm, t->box_lo, t->box_hi, (float)p->theta,
-	                     (int)p->leaf_size, t->rec_a, t->rec_b, s))
+	if (sfd_lbvh_records(N, t->node_lr, t->node_range, t->node_cm, t->node_cell, (float)p->theta, (int)p->leaf_size,
+	                     t->rec_b, s))
 		return -1;
-	return sfd_lbvh_walk(t->body, t->val0, N, t->rec_a, t->rec_b, (float)(p->epsilon * p->epsilon), p->G,
-	                     p->acc_mode == SF_ACC_PHYSICAL, acc, pot, t->counters, t->gstack, t->next_group, s);
+	return sfd_lbvh_walk(t->body, t->val0, N, t->node_cm, t->rec_b, (float)(p->epsilon * p->epsilon), p->G,
+	                     p->acc_mode == SF_ACC_PHYSICAL, acc, pot, t->counters, t->gstack, s);
 }
 
 int sf_lbvh_export_impl(sf_lbvh* t, hipStream_t s, unsigned long long* keys, unsigned int* order,
