@@ -406,9 +406,45 @@ __global__ __launch_bounds__(kBlock) void k_records(int N, const int2* __restric
 	rec_b[n] = make_int4(__float_as_int(rc * rc), leafy ? first : l, leafy ? count : r, leafy ? count : -count);
 }
 
+// Walk groups: the maximal subtrees holding at most `cap` particles.  They
+// partition the sorted order into runs of at most cap particles that each lie
+// inside one Morton cell; start[p] = 1 where a run begins, start[N] = 1 ends
+// the last run.  `start` is zero beforehand.
+__global__ __launch_bounds__(kBlock) void k_group_flags(int N, const int2* __restrict__ node_lr,
+                                                        const int2* __restrict__ node_range, int cap,
+                                                        unsigned* __restrict__ start) {
+	const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+	if (i == 0) {
+		start[N] = 1u;
+		if (N <= cap) start[0] = 1u;   // the root (a single leaf when N == 1) is the only group
+	}
+	if (i >= N - 1) return;
+	if (node_range[i].y <= cap) return;
+	const int2 lr = node_lr[i];
+	const int child[2] = {lr.x, lr.y};
+	for (int k = 0; k < 2; ++k) {
+		const int c = child[k];
+		const int2 rg = c >= N - 1 ? make_int2(c - (N - 1), 1) : node_range[c];
+		if (rg.y <= cap) start[rg.x] = 1u;
+	}
+}
+
 }  // namespace
 
 extern "C" {
+
+unsigned sfd_lbvh_group_flag_len(unsigned N) { return (N + 63u) / 64u * 64u + 64u; }
+
+int sfd_lbvh_groups(unsigned N, const void* node_lr, const void* node_range, int cap, unsigned* start,
+                    hipStream_t s) {
+	if (N == 0 || cap < 1 || cap > 64) return -1;
+	const unsigned len = sfd_lbvh_group_flag_len(N);
+	hipLaunchKernelGGL(k_fill_u32, dim3((len + kBlock - 1) / kBlock), dim3(kBlock), 0, s, start, len, 0u);
+	const unsigned threads = N > 1 ? N - 1 : 1;
+	hipLaunchKernelGGL(k_group_flags, dim3((threads + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (int)N,
+	                   static_cast<const int2*>(node_lr), static_cast<const int2*>(node_range), cap, start);
+	return sf_hip_check(hipGetLastError(), "sfd_lbvh_groups", "launch");
+}
 
 unsigned sfd_lbvh_bbox_blocks(void) { return 256u; }
 

@@ -1,5 +1,9 @@
-// lbvh_walk.hip -- the SF_TREE_LBVH walk: one wavefront per group of 64
-// consecutive sorted particles.
+// lbvh_walk.hip -- the SF_TREE_LBVH walk: one wavefront per walk group, a
+// maximal subtree of at most 64 particles (a run of consecutive sorted
+// particles inside one Morton cell).  Groups are not fixed runs of 64 sorted
+// particles: a run that crosses a high-level Morton boundary (the octant
+// planes through the centre of a cluster) has a bounding box the size of the
+// whole core and would open nearly every node.
 //
 // Each step pops up to 64 pending nodes, one per lane, and classifies them
 // against the group's bounding box: a monopole "cell" when the box stays
@@ -12,7 +16,8 @@
 // so the tree logic and memory latency are amortised over 64 interactions.
 //
 // The grid is persistent and sized to the wavefronts resident at once; each
-// wavefront takes every (grid size)-th group and owns a node stack in global
+// wavefront takes every (grid size)-th run of 64 sorted positions, walks the
+// groups that start in it, and owns a node stack in global
 // memory (deep enough that the 64-wide pops never have to narrow).  The self
 // pair is included in leaf sums, as in the reference's PP sum.
 #include <hip/hip_runtime.h>
@@ -86,11 +91,12 @@ __device__ __forceinline__ void eval_list(Acc& a, float xi, float yi, float zi, 
 }
 
 __global__ __launch_bounds__(kBlock) void k_walk(const float4* __restrict__ body, const unsigned* __restrict__ order,
-                                                 int N, const float4* __restrict__ rec_a,
+                                                 int N, const unsigned* __restrict__ group_start,
+                                                 const float4* __restrict__ rec_a,
                                                  const int4* __restrict__ rec_b, float eps2, double G, int acc_phys,
                                                  float* __restrict__ acc, float* __restrict__ pot,
                                                  unsigned long long* __restrict__ counters, int* __restrict__ gstack,
-                                                 unsigned n_groups) {
+                                                 unsigned n_chunks) {
 	__shared__ float4 s_cell[kWaves][kList];
 	__shared__ int2 s_leaf[kWaves][kList];
 	__shared__ int s_pid[kWaves][kList];
@@ -107,131 +113,147 @@ __global__ __launch_bounds__(kBlock) void k_walk(const float4* __restrict__ body
 	unsigned long long n_cell_tot = 0, n_part_tot = 0, n_batch = 0;
 	unsigned bad = 0;
 
-	// static round-robin over groups: a single shared work counter would make
+	// static round-robin over chunks: a single shared work counter would make
 	// every wavefront wait on device-scope atomics that serialise across XCDs
 	const unsigned n_slots = gridDim.x * kWaves;
-	for (unsigned g = blockIdx.x * kWaves + w; g < n_groups; g += n_slots) {
-		const int base = (int)g * 64;
-		const int p = base + (int)lane;
-		const bool live = p < N;
-		const float4 me = body[live ? p : N - 1];
-		const float xi = me.x, yi = me.y, zi = me.z;
-		const float gx0 = wave_min(xi), gy0 = wave_min(yi), gz0 = wave_min(zi);
-		const float gx1 = wave_max(xi), gy1 = wave_max(yi), gz1 = wave_max(zi);
+	for (unsigned c = blockIdx.x * kWaves + w; c < n_chunks; c += n_slots) {
+		// the groups that start in sorted positions [64c, 64c + 64); each ends at
+		// the next start flag, at most 64 positions later (start[N] is set)
+		const unsigned at = 64u * c + lane;
+		const unsigned long long e1 = __ballot(group_start[at] != 0u);
+		const unsigned long long e2 = __ballot(group_start[at + 64u] != 0u);
+		unsigned long long rem = e1 & __ballot(at < (unsigned)N);
+		while (rem) {
+			const int s0 = __builtin_ctzll(rem);
+			rem &= rem - 1ull;
+			const unsigned long long above = s0 < 63 ? e1 & (~0ull << (s0 + 1)) : 0ull;
+			const int e0 = above ? __builtin_ctzll(above) : (e2 ? 64 + __builtin_ctzll(e2) : 128);
+			const int base = 64 * (int)c + s0;
+			int cnt = e0 - s0;
+			if (cnt > 64 || cnt > N - base) {   // only a malformed start array gets here
+				bad = 1u;
+				cnt = min(64, N - base);
+			}
+			const int p = base + (int)lane;
+			const bool live = (int)lane < cnt;
+			const float4 me = body[live ? p : base];
+			const float xi = me.x, yi = me.y, zi = me.z;
+			const float gx0 = wave_min(xi), gy0 = wave_min(yi), gz0 = wave_min(zi);
+			const float gx1 = wave_max(xi), gy1 = wave_max(yi), gz1 = wave_max(zi);
 
-		Acc a;
-		a.tx = a.ty = a.tz = a.tp = a.sx = a.sy = a.sz = a.sp = a.cx = a.cy = a.cz = a.cp = 0.0f;
-		unsigned long long n_cell = 0, n_part = 0;
-		unsigned popped = 0;
-		int sp = 1, nc = 0, nl = 0, npid = 0;
-		if (lane == 0) stack[0] = 0;   // root: internal node 0, or the only leaf when N == 1
-		wave_fence();
+			Acc a;
+			a.tx = a.ty = a.tz = a.tp = a.sx = a.sy = a.sz = a.sp = a.cx = a.cy = a.cz = a.cp = 0.0f;
+			unsigned long long n_cell = 0, n_part = 0;
+			unsigned popped = 0;
+			int sp = 1, nc = 0, nl = 0, npid = 0;
+			if (lane == 0) stack[0] = 0;   // root: internal node 0, or the only leaf when N == 1
+			wave_fence();
 
-		while (sp > 0 || nc > 0 || nl > 0 || npid > 0) {
-			if (sp > 0) {
-				const int k = (sp + 64 <= kStack - kHead) ? min(64, sp) : 1;
-				int node = -1;
-				if ((int)lane < k) node = stack[sp - 1 - (int)lane];
-				sp -= k;
-				popped += (unsigned)k;
-				++n_batch;
-				wave_fence();
-				if (node >= n_nodes) {
-					node = -1;   // only a malformed tree gets here
-					bad = 1u;
-				}
-				int cls = 0;   // 0 none, 1 cell, 2 leaf, 3 open
-				float4 ra = make_float4(0.f, 0.f, 0.f, 0.f);
-				int4 rb = make_int4(0, 0, 0, 0);
-				if (node >= 0) {
-					ra = rec_a[node];
-					rb = rec_b[node];
-					const float dx = fmaxf(fmaxf(gx0 - ra.x, ra.x - gx1), 0.0f);
-					const float dy = fmaxf(fmaxf(gy0 - ra.y, ra.y - gy1), 0.0f);
-					const float dz = fmaxf(fmaxf(gz0 - ra.z, ra.z - gz1), 0.0f);
-					const float d2 = dx * dx + dy * dy + dz * dz;
-					if (d2 > __int_as_float(rb.x) && abs(rb.w) > 1) cls = 1;
-					else if (rb.w > 0) cls = 2;   // (first, count) record
-					else cls = 3;
-				}
-				const unsigned long long m_cell = __ballot(cls == 1);
-				const unsigned long long m_leaf = __ballot(cls == 2);
-				const unsigned long long m_open = __ballot(cls == 3);
-				if (cls == 1) cells[nc + __popcll(m_cell & lt)] = ra;
-				if (cls == 2) leaves[nl + __popcll(m_leaf & lt)] = make_int2(rb.y, rb.z);
-				if (cls == 3) {
-					const int at = sp + 2 * __popcll(m_open & lt);
-					if (at + 1 < kStack) {   // always true: see kHead
-						stack[at] = rb.y;
-						stack[at + 1] = rb.z;
-					} else {
+			while (sp > 0 || nc > 0 || nl > 0 || npid > 0) {
+				if (sp > 0) {
+					const int k = (sp + 64 <= kStack - kHead) ? min(64, sp) : 1;
+					int node = -1;
+					if ((int)lane < k) node = stack[sp - 1 - (int)lane];
+					sp -= k;
+					popped += (unsigned)k;
+					++n_batch;
+					wave_fence();
+					if (node >= n_nodes) {
+						node = -1;   // only a malformed tree gets here
 						bad = 1u;
 					}
+					int cls = 0;   // 0 none, 1 cell, 2 leaf, 3 open
+					float4 ra = make_float4(0.f, 0.f, 0.f, 0.f);
+					int4 rb = make_int4(0, 0, 0, 0);
+					if (node >= 0) {
+						ra = rec_a[node];
+						rb = rec_b[node];
+						const float dx = fmaxf(fmaxf(gx0 - ra.x, ra.x - gx1), 0.0f);
+						const float dy = fmaxf(fmaxf(gy0 - ra.y, ra.y - gy1), 0.0f);
+						const float dz = fmaxf(fmaxf(gz0 - ra.z, ra.z - gz1), 0.0f);
+						const float d2 = dx * dx + dy * dy + dz * dz;
+						if (d2 > __int_as_float(rb.x) && abs(rb.w) > 1) cls = 1;
+						else if (rb.w > 0) cls = 2;   // (first, count) record
+						else cls = 3;
+					}
+					const unsigned long long m_cell = __ballot(cls == 1);
+					const unsigned long long m_leaf = __ballot(cls == 2);
+					const unsigned long long m_open = __ballot(cls == 3);
+					if (cls == 1) cells[nc + __popcll(m_cell & lt)] = ra;
+					if (cls == 2) leaves[nl + __popcll(m_leaf & lt)] = make_int2(rb.y, rb.z);
+					if (cls == 3) {
+						const int at = sp + 2 * __popcll(m_open & lt);
+						if (at + 1 < kStack) {   // always true: see kHead
+							stack[at] = rb.y;
+							stack[at + 1] = rb.z;
+						} else {
+							bad = 1u;
+						}
+					}
+					nc += __popcll(m_cell);
+					nl += __popcll(m_leaf);
+					sp = min(sp + 2 * __popcll(m_open), kStack);
+					wave_fence();
+					if (popped > budget) {   // a tree visits each node at most once per group
+						bad = 1u;
+						sp = 0;
+					}
 				}
-				nc += __popcll(m_cell);
-				nl += __popcll(m_leaf);
-				sp = min(sp + 2 * __popcll(m_open), kStack);
-				wave_fence();
-				if (popped > budget) {   // a tree visits each node at most once per group
-					bad = 1u;
-					sp = 0;
+				const bool drain = sp == 0;
+				// monopoles
+				if (nc >= 64 || (drain && nc > 0)) {
+					const int take = min(nc, 64);
+					eval_list(a, xi, yi, zi, cells, take, eps2);
+					n_cell += (unsigned long long)take;
+					wave_fence();
+					if ((int)lane < nc - take) cells[lane] = cells[take + lane];
+					nc -= take;
+					wave_fence();
 				}
-			}
-			const bool drain = sp == 0;
-			// monopoles
-			if (nc >= 64 || (drain && nc > 0)) {
-				const int take = min(nc, 64);
-				eval_list(a, xi, yi, zi, cells, take, eps2);
-				n_cell += (unsigned long long)take;
-				wave_fence();
-				if ((int)lane < nc - take) cells[lane] = cells[take + lane];
-				nc -= take;
-				wave_fence();
-			}
-			// leaves: expand into particle ids, then gather and sum 64 at a time
-			if (nl >= 64 || (drain && (nl > 0 || npid > 0))) {
-				int q = 0;
-				while (q < nl || (drain && npid > 0)) {
-					while (q < nl && npid < 64) {
-						const int2 lf = leaves[q];
-						if ((int)lane < lf.y) pid[npid + (int)lane] = lf.x + (int)lane;
-						npid += lf.y;
-						++q;
+				// leaves: expand into particle ids, then gather and sum 64 at a time
+				if (nl >= 64 || (drain && (nl > 0 || npid > 0))) {
+					int q = 0;
+					while (q < nl || (drain && npid > 0)) {
+						while (q < nl && npid < 64) {
+							const int2 lf = leaves[q];
+							if ((int)lane < lf.y) pid[npid + (int)lane] = lf.x + (int)lane;
+							npid += lf.y;
+							++q;
+						}
+						wave_fence();
+						if (npid >= 64 || (drain && q == nl && npid > 0)) {
+							const int take = min(npid, 64);
+							if ((int)lane < take) part[lane] = body[pid[lane]];
+							wave_fence();
+							eval_list(a, xi, yi, zi, part, take, eps2);
+							n_part += (unsigned long long)take;
+							wave_fence();
+							if ((int)lane < npid - take) pid[lane] = pid[take + (int)lane];
+							npid -= take;
+							wave_fence();
+						} else {
+							break;
+						}
 					}
 					wave_fence();
-					if (npid >= 64 || (drain && q == nl && npid > 0)) {
-						const int take = min(npid, 64);
-						if ((int)lane < take) part[lane] = body[pid[lane]];
-						wave_fence();
-						eval_list(a, xi, yi, zi, part, take, eps2);
-						n_part += (unsigned long long)take;
-						wave_fence();
-						if ((int)lane < npid - take) pid[lane] = pid[take + (int)lane];
-						npid -= take;
-						wave_fence();
-					} else {
-						break;
-					}
+					if ((int)lane < nl - q) leaves[lane] = leaves[q + (int)lane];
+					nl -= q;
+					wave_fence();
 				}
-				wave_fence();
-				if ((int)lane < nl - q) leaves[lane] = leaves[q + (int)lane];
-				nl -= q;
-				wave_fence();
 			}
+			if (live) {
+				const unsigned i = order[p];
+				const double sx = (double)a.sx - (double)a.cx, sy = (double)a.sy - (double)a.cy,
+				             sz = (double)a.sz - (double)a.cz, sp2 = (double)a.sp - (double)a.cp;
+				const double f = acc_phys ? G : G * (double)me.w;
+				acc[3u * i + 0u] = (float)(f * sx);
+				acc[3u * i + 1u] = (float)(f * sy);
+				acc[3u * i + 2u] = (float)(f * sz);
+				pot[i] = (float)(-G * sp2);
+			}
+			n_cell_tot += n_cell * (unsigned long long)cnt;
+			n_part_tot += n_part * (unsigned long long)cnt;
 		}
-		if (live) {
-			const unsigned i = order[p];
-			const double sx = (double)a.sx - (double)a.cx, sy = (double)a.sy - (double)a.cy,
-			             sz = (double)a.sz - (double)a.cz, sp2 = (double)a.sp - (double)a.cp;
-			const double f = acc_phys ? G : G * (double)me.w;
-			acc[3u * i + 0u] = (float)(f * sx);
-			acc[3u * i + 1u] = (float)(f * sy);
-			acc[3u * i + 2u] = (float)(f * sz);
-			pot[i] = (float)(-G * sp2);
-		}
-		const unsigned long long nlive = (unsigned long long)min(64, N - base);
-		n_cell_tot += n_cell * nlive;
-		n_part_tot += n_part * nlive;
 	}
 	const bool any_bad = __ballot(bad != 0u) != 0ull;
 	if (counters && lane == 0) {
@@ -270,19 +292,19 @@ extern "C" {
 
 size_t sfd_lbvh_walk_stack_bytes(void) { return (size_t)resident_blocks() * kWaves * kStack * sizeof(int); }
 
-int sfd_lbvh_walk(const void* body, const unsigned* order, unsigned N, const void* node_cm, const void* rec_b,
-                  float eps2, double G, int acc_phys, float* acc, float* pot, unsigned long long* counters,
-                  int* gstack, hipStream_t s) {
+int sfd_lbvh_walk(const void* body, const unsigned* order, unsigned N, const unsigned* group_start,
+                  const void* node_cm, const void* rec_b, float eps2, double G, int acc_phys, float* acc, float* pot,
+                  unsigned long long* counters, int* gstack, hipStream_t s) {
 	if (N == 0) return 0;
-	if (!counters || !gstack) return -1;
-	const unsigned groups = (N + 63) / 64;
-	unsigned blocks = (groups + kWaves - 1) / kWaves;
+	if (!counters || !gstack || !group_start) return -1;
+	const unsigned chunks = (N + 63) / 64;   // group_start holds 64 * (chunks + 1) flags
+	unsigned blocks = (chunks + kWaves - 1) / kWaves;
 	if (blocks > resident_blocks()) blocks = resident_blocks();   // gstack holds resident_blocks() slots
 	hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, s, counters);
 	const int tok = sf_prof_begin("lbvh_walk", s);
 	hipLaunchKernelGGL(k_walk, dim3(blocks), dim3(kBlock), 0, s, static_cast<const float4*>(body), order, (int)N,
-	                   static_cast<const float4*>(node_cm), static_cast<const int4*>(rec_b), eps2, G, acc_phys, acc,
-	                   pot, counters, gstack, groups);
+	                   group_start, static_cast<const float4*>(node_cm), static_cast<const int4*>(rec_b), eps2, G,
+	                   acc_phys, acc, pot, counters, gstack, chunks);
 	sf_prof_end(tok, s);
 	return sf_hip_check(hipGetLastError(), "sfd_lbvh_walk", "launch");
 }

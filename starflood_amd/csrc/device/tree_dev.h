@@ -55,9 +55,16 @@ int sfd_lbvh_records(unsigned int N, const void* node_lr, const void* node_range
    particle interactions, wavefronts that met a malformed tree, node batches.
    gstack: sfd_lbvh_walk_stack_bytes() of scratch. */
 size_t sfd_lbvh_walk_stack_bytes(void);
-int sfd_lbvh_walk(const void* body, const unsigned int* order, unsigned int N, const void* node_cm, const void* rec_b,
-                  float eps2, double G, int acc_phys, float* acc, float* pot, unsigned long long* counters,
-                  int* gstack, hipStream_t s);
+int sfd_lbvh_walk(const void* body, const unsigned int* order, unsigned int N, const unsigned int* group_start,
+                  const void* node_cm, const void* rec_b, float eps2, double G, int acc_phys, float* acc, float* pot,
+                  unsigned long long* counters, int* gstack, hipStream_t s);
+
+/* walk groups: start[p] = 1 where a maximal subtree of at most cap (<= 64)
+   particles begins in sorted order, and start[N] = 1; start holds
+   sfd_lbvh_group_flag_len(N) entries (lbvh.hip) */
+unsigned int sfd_lbvh_group_flag_len(unsigned int N);
+int sfd_lbvh_groups(unsigned int N, const void* node_lr, const void* node_range, int cap, unsigned int* start,
+                    hipStream_t s);
 
 #ifdef __cplusplus
 }

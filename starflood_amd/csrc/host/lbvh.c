@@ -8,6 +8,10 @@
 #include "../device/tree_dev.h"
 #include "sf_host.h"
 
+/* Walk group size cap: groups are the maximal subtrees with at most this many
+   particles (one wavefront each). */
+#define SF_LBVH_GROUP 64
+
 struct sf_lbvh {
 	unsigned int n_alloc;     /* N the buffers are sized for */
 	unsigned int N;           /* particles in the last build */
@@ -30,12 +34,13 @@ struct sf_lbvh {
 	unsigned long long* counters;   /* 4 */
 	unsigned long long* h_counters; /* pinned */
 	int* gstack;                    /* walk node stacks, one per wavefront slot */
+	unsigned int* group_start;      /* walk group starts in sorted order */
 };
 
 static void free_buffers(sf_lbvh* t) {
 	void* ptrs[] = {t->key0, t->key1, t->val0, t->val1, t->hist, t->bbox_part, t->box, t->body, t->node_lr,
 	                t->node_range, t->prefix, t->tile_sum, t->node_cm, t->node_cell, t->rec_b, t->counters,
-	                t->gstack};
+	                t->gstack, t->group_start};
 	for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k)
 		if (ptrs[k]) hipFree(ptrs[k]);
 	if (t->h_counters) hipHostFree(t->h_counters);
@@ -81,6 +86,7 @@ static int ensure(sf_lbvh** st, unsigned int N) {
 	rc |= alloc(&t->rec_b, nn * 16u);
 	rc |= alloc((void**)&t->counters, 4u * 8u);
 	rc |= alloc((void**)&t->gstack, sfd_lbvh_walk_stack_bytes());
+	rc |= alloc((void**)&t->group_start, (size_t)sfd_lbvh_group_flag_len(N) * 4u);
 	if (!rc && hipHostMalloc((void**)&t->h_counters, 4u * 8u, hipHostMallocDefault) != hipSuccess) rc = -1;
 	if (rc) {
 		sf_error("tree_build", "device allocation for an LBVH of %u particles failed", N);
@@ -108,6 +114,7 @@ int sf_lbvh_build(sf_lbvh** st, unsigned int N, const float* pos, const float* m
 	if (sfd_lbvh_topology(pos, mas, t->key0, t->val0, N, t->box, t->body, t->node_lr, t->node_range, t->prefix,
 	                      t->tile_sum, t->node_cm, t->node_cell, s))
 		return -1;
+	if (sfd_lbvh_groups(N, t->node_lr, t->node_range, SF_LBVH_GROUP, t->group_start, s)) return -1;
 	sf_prof_end(tok, s);
 	return 0;
 }
@@ -124,8 +131,8 @@ int sf_lbvh_walk(sf_lbvh* t, unsigned int N, const float* pos, const float* mas,
 	if (sfd_lbvh_records(N, t->node_lr, t->node_range, t->node_cm, t->node_cell, (float)p->theta, (int)p->leaf_size,
 	                     t->rec_b, s))
 		return -1;
-	return sfd_lbvh_walk(t->body, t->val0, N, t->node_cm, t->rec_b, (float)(p->epsilon * p->epsilon), p->G,
-	                     p->acc_mode == SF_ACC_PHYSICAL, acc, pot, t->counters, t->gstack, s);
+	return sfd_lbvh_walk(t->body, t->val0, N, t->group_start, t->node_cm, t->rec_b, (float)(p->epsilon * p->epsilon),
+	                     p->G, p->acc_mode == SF_ACC_PHYSICAL, acc, pot, t->counters, t->gstack, s);
 }
 
 int sf_lbvh_export_impl(sf_lbvh* t, hipStream_t s, unsigned long long* keys, unsigned int* order,
