@@ -154,6 +154,18 @@ def sph_ref(pos, mas, rad, acc_in, eps=1e-2, kahan=1):
     return rho, prs, acc
 
 
+def sph_cubic_f64(pos, vel, mas, h, u, gamma):
+    """Cubic-spline ideal-gas SPH in FP64: rho, P, SPH acceleration (physical
+    units) and du/dt."""
+    d = lambda a: np.ascontiguousarray(a, np.float64).ravel()
+    pos, vel, mas, h, u = d(pos), d(vel), d(mas), d(h), d(u)
+    N = len(mas)
+    rho, prs, dudt = np.zeros(N), np.zeros(N), np.zeros(N)
+    acc = np.zeros(3 * N)
+    lib().orc_sph_cubic_f64(N, dp(pos), dp(vel), dp(mas), dp(h), dp(u), gamma, dp(rho), dp(prs), dp(acc), dp(dudt))
+    return rho, prs, acc.reshape(N, 3), dudt
+
+
 def tree_ref(pos, mas, step_number, max_nodes=262144, depth_max=16):
     """Bounds, build and node array of the reference octree."""
     pos = np.ascontiguousarray(pos, np.float32)

@@ -165,7 +165,7 @@ void* sf_device_find(const sim_t* sim_ptr, enum sim_conf parameter_enum) {
 	return sf_blk(c->d_mem, c->N, e);
 }
 
-static int ensure_u(sf_ctx* c) {
+int sf_ctx_ensure_u(sf_ctx* c) {
 	if (!c->d_u) {
 		SF_HIP(hipMalloc((void**)&c->d_u, ((size_t)c->N + 1u) * sizeof(float)), "sf_device_ext");
 		SF_HIP(hipMemsetAsync(c->d_u, 0, ((size_t)c->N + 1u) * sizeof(float), c->stream), "sf_device_ext");
@@ -183,10 +183,11 @@ void* sf_device_ext(const sim_t* sim_ptr, int which) {
 	if (!c) return NULL;
 	switch (which) {
 	case SF_EXT_U:
-		if (ensure_u(c)) return NULL;
+		if (sf_ctx_ensure_u(c)) return NULL;
+		c->have_u = 1;   /* the caller may write u through the pointer */
 		return c->d_u;
 	case SF_EXT_DUDT:
-		if (ensure_u(c)) return NULL;
+		if (sf_ctx_ensure_u(c)) return NULL;
 		return c->d_dudt;
 	case SF_EXT_X64: return c->d_x64;
 	default: sf_error("sf_device_ext", "unknown array %d", which); return NULL;

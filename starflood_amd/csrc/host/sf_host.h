@@ -16,10 +16,14 @@ int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number);
 /* tree.c: after an LBVH solve on stream s, fail if the walk met a malformed
    tree (synchronises s; no-op in reference mode) */
 int sf_tree_check(tree_t* tree, hipStream_t s);
-/* sph.c: density, EOS and pressure force, added into acc */
+/* sph.c: density, EOS and pressure force, added into acc.  For the cubic
+   model sf_sph_prepare packs the particle records first, while acc still
+   holds the previous forces: velocities and u are predicted pred_dt ahead. */
+int sf_sph_prepare(sf_ctx* c, float pred_dt);
 int sf_solve_sph(sf_ctx* c);
-/* simulation.c: the solver sequence of sim_solv on device state */
-int sf_solve_all(sf_ctx* c, tree_t* tree, unsigned int step_number);
+/* simulation.c: the solver sequence of sim_solv on device state
+   (sph_pred_dt: see sf_sph_prepare) */
+int sf_solve_all(sf_ctx* c, tree_t* tree, unsigned int step_number, float sph_pred_dt);
 
 /* lbvh.c: SF_TREE_LBVH build (allocates *st on first use) and walk */
 typedef struct sf_lbvh sf_lbvh;

@@ -140,9 +140,10 @@ static int download_solver_outputs(sf_ctx* c) {
 }
 
 /* Solver sequence of src/simulation.c:231-345 on device state. */
-int sf_solve_all(sf_ctx* c, tree_t* tree, unsigned int step_number) {
+int sf_solve_all(sf_ctx* c, tree_t* tree, unsigned int step_number, float sph_pred_dt) {
 	const sf_params_t* p = sf_params();
 	int rc = 0;
+	if (p->enable_sph && sf_sph_prepare(c, sph_pred_dt)) return -1;
 	if (p->enable_tree) {
 		if (sf_solve_gravity_tree(c, tree, step_number)) rc = -1;
 	} else {
@@ -162,7 +163,7 @@ int sim_solv(sim_t* SF_RESTRICT sim_ptr) {
 	if (!c) return STARFLOOD_FAILURE;
 	int rc = 0;
 	if (upload_solver_inputs(c)) return STARFLOOD_FAILURE;
-	if (sf_solve_all(c, &sim_ptr->tree, sim_ptr->step_number)) rc = -1;
+	if (sf_solve_all(c, &sim_ptr->tree, sim_ptr->step_number, 0.0f)) rc = -1;
 	if (download_solver_outputs(c)) rc = -1;
 	if (sf_finish(c, "sim_solv")) rc = -1;
 	if (sf_params()->enable_tree && sf_tree_check(&sim_ptr->tree, c->stream)) rc = -1;
@@ -194,6 +195,9 @@ int sim_step(sim_t* SF_RESTRICT sim_ptr) {
 	const float half_dt = 0.5f * dt;       /* (real)0.5 * dt */
 	const int exact = p->precision == SF_PRECISION_EXACT;
 	const int timing = p->logging && g_log_timings;
+	/* the cubic SPH model carries u: it is kicked with v, and the solve
+	   predicts v and u to the end of the step from the previous forces */
+	const int cubic = p->enable_sph && p->sph_model == SF_SPH_CUBIC_IDEAL;
 	int rc = 0;
 
 	if (timing && !g_tev_ready) {
@@ -206,6 +210,7 @@ int sim_step(sim_t* SF_RESTRICT sim_ptr) {
 	if (sf_upload_blocks(c, SF_E_POS, 10u)) return STARFLOOD_FAILURE;
 	if (p->enable_sph && sf_upload_blocks(c, SF_E_RAD, 1u)) return STARFLOOD_FAILURE;
 
+	if (cubic && (sf_ctx_ensure_u(c) || sfd_sph_u_kick(c->d_u, c->d_dudt, N, half_dt, s))) return STARFLOOD_FAILURE;
 	if (timing) tev_record(0, s);
 	if (exact) {
 		if (sfd_kick_exact(vel, acc, n3, half_dt, s)) return STARFLOOD_FAILURE;    /* kick_0 */
@@ -218,7 +223,7 @@ int sim_step(sim_t* SF_RESTRICT sim_ptr) {
 	}
 	if (timing) tev_record(2, s);
 
-	if (sf_solve_all(c, &sim_ptr->tree, sim.step_number)) {
+	if (sf_solve_all(c, &sim_ptr->tree, sim.step_number, cubic ? half_dt : 0.0f)) {
 		fprintf(stderr, "%s error: %s failed.\n", "sim_step()", "sim_solv()");
 		rc = -1;
 	}
@@ -229,6 +234,7 @@ int sim_step(sim_t* SF_RESTRICT sim_ptr) {
 	} else {
 		if (sfd_kick_fast(vel, acc, n3, half_dt, s)) return STARFLOOD_FAILURE;
 	}
+	if (cubic && sfd_sph_u_kick(c->d_u, c->d_dudt, N, half_dt, s)) return STARFLOOD_FAILURE;
 	if (timing) tev_record(4, s);
 	if (sfd_energy_terms(vel, mas, pot, N, ken, pen, s)) return STARFLOOD_FAILURE;
 

@@ -58,6 +58,8 @@ sf_ctx* sf_ctx_create(void* host_mem, unsigned int N, unsigned int dof);
 void sf_ctx_destroy(sf_ctx* c);
 /* Ensure the PP partial-sum scratch holds `slices` slices. */
 int sf_ctx_reserve_part(sf_ctx* c, unsigned int slices);
+/* allocates (zeroed) d_u and d_dudt on first use */
+int sf_ctx_ensure_u(sf_ctx* c);
 /* The library's stream for work not tied to a simulation (lazy). */
 hipStream_t sf_default_stream(void);
 
@@ -121,6 +123,8 @@ int sfd_x64_refresh(const float* pos, double* x64, unsigned int n3, hipStream_t 
 int sfd_kick_drift_fast(float* vel, const float* acc, double* x64, float* pos, unsigned int n3,
                         float half_dt, double dt, hipStream_t s);
 int sfd_kick_fast(float* vel, const float* acc, unsigned int n3, float half_dt, hipStream_t s);
+/* sph.hip: u += half_dt du/dt (cubic SPH model) */
+int sfd_sph_u_kick(float* u, const float* dudt, unsigned int N, float half_dt, hipStream_t s);
 
 /* Deterministic FP64 reductions over the state: fills out[0..SF_RED_VALUES)
    with T, V, W, U, Px, Py, Pz, Lx, Ly, Lz.  x64 may be NULL (use pos).

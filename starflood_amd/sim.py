@@ -124,6 +124,23 @@ class Simulation:
             raise RuntimeError(L.last_error())
         return p
 
+    def set_u(self, u) -> None:
+        """Specific internal energy of the cubic SPH model (device array)."""
+        u = np.ascontiguousarray(np.broadcast_to(np.asarray(u, np.float32), (self.N,)))
+        L.check(self.lib.sf_ext_copy(C.byref(self.sim), L.SF_EXT_U, u.ctypes.data, L.SF_SYNC_TO_DEVICE),
+                "sf_ext_copy")
+
+    def _get_ext(self, which: int) -> np.ndarray:
+        out = np.zeros(self.N, np.float32)
+        L.check(self.lib.sf_ext_copy(C.byref(self.sim), which, out.ctypes.data, L.SF_SYNC_TO_HOST), "sf_ext_copy")
+        return out
+
+    def get_u(self) -> np.ndarray:
+        return self._get_ext(L.SF_EXT_U)
+
+    def get_dudt(self) -> np.ndarray:
+        return self._get_ext(L.SF_EXT_DUDT)
+
     def stats(self) -> dict:
         s = L.Stats()
         L.check(self.lib.sf_get_stats(C.byref(self.sim), C.byref(s)), "sf_get_stats")
