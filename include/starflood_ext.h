@@ -194,8 +194,10 @@ int sf_pp_direct_f64(const sim_t* sim_ptr, const unsigned int* idx, unsigned int
 typedef struct sf_tape sf_tape;
 /* Run `nsteps` KDK steps of PP gravity from (x0, v0) with the current
    epsilon, G, dt and acc_mode; write the final state to (xN, vN) and return
-   a tape for sf_adjoint_backward.  `stream` is a hipStream_t (NULL: the
-   library stream). */
+   a tape for sf_adjoint_backward.  `stream` is the caller's hipStream_t
+   (NULL: HIP's default null stream); the work is ordered on it and these
+   calls return without waiting for it, except sf_force_dev and
+   sf_force_vjp_dev, which synchronise it. */
 int sf_adjoint_forward(unsigned int N, const float* x0, const float* v0, const float* m,
                        unsigned int nsteps, unsigned int ckpt_every,
                        float* xN, float* vN, void* stream, sf_tape** tape_out);

@@ -166,6 +166,39 @@ def sph_cubic_f64(pos, vel, mas, h, u, gamma):
     return rho, prs, acc.reshape(N, 3), dudt
 
 
+def _d64(a):
+    return np.ascontiguousarray(a, np.float64).ravel().copy()
+
+
+def force_f64(x, m, eps, G=1.0, acc_phys=1):
+    x, m = _d64(x), _d64(m)
+    acc = np.zeros_like(x)
+    lib().orc_force_f64(len(m), dp(x), dp(m), eps, G, acc_phys, dp(acc))
+    return acc.reshape(-1, 3)
+
+
+def force_vjp_f64(x, m, mu, eps, G=1.0, acc_phys=1):
+    x, m, mu = _d64(x), _d64(m), _d64(mu)
+    g = np.zeros_like(x)
+    lib().orc_force_vjp_f64(len(m), dp(x), dp(m), eps, G, acc_phys, dp(mu), dp(g))
+    return g.reshape(-1, 3)
+
+
+def kdk_f64(x, v, m, eps, G, dt, acc_phys, nsteps):
+    x, v, m = _d64(x), _d64(v), _d64(m)
+    lib().orc_kdk_f64(len(m), dp(x), dp(v), dp(m), eps, G, dt, acc_phys, nsteps)
+    return x.reshape(-1, 3), v.reshape(-1, 3)
+
+
+def kdk_adjoint_f64(x0, v0, m, eps, G, dt, acc_phys, nsteps, target):
+    """(L, dL/dx0, dL/dv0) for L = mean((x_N - target)^2)."""
+    x0, v0, m, target = _d64(x0), _d64(v0), _d64(m), _d64(target)
+    gx, gv = np.zeros_like(x0), np.zeros_like(v0)
+    L = lib().orc_kdk_adjoint_f64(len(m), dp(x0), dp(v0), dp(m), eps, G, dt, acc_phys, nsteps, dp(target), dp(gx),
+                                  dp(gv))
+    return L, gx.reshape(-1, 3), gv.reshape(-1, 3)
+
+
 def tree_ref(pos, mas, step_number, max_nodes=262144, depth_max=16):
     """Bounds, build and node array of the reference octree."""
     pos = np.ascontiguousarray(pos, np.float32)

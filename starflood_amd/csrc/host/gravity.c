@@ -49,6 +49,16 @@ static int pp_on(const sf_params_t* p, unsigned int N, unsigned int n_pad, const
 	return sfd_pp_fast(body, N, n_pad, eps2_f32(p->epsilon), p->G, phys, *part, slices, acc, pot, s);
 }
 
+unsigned int sf_pp_slice_count(unsigned int n_pad) { return pp_slices(n_pad); }
+
+int sf_pp_force_arrays(const sf_params_t* p, unsigned int N, unsigned int n_pad, const float* pos, const float* mas,
+                       float* body, float* part, unsigned int part_slices, float* acc, float* pot, hipStream_t s) {
+	if (part_slices < pp_slices(n_pad)) return -1;
+	if (sfd_pack_bodies(pos, mas, N, n_pad, body, s)) return -1;
+	return sfd_pp_fast(body, N, n_pad, eps2_f32(p->epsilon), p->G, p->acc_mode == SF_ACC_PHYSICAL, part,
+	                   pp_slices(n_pad), acc, pot, s);
+}
+
 int sf_solve_gravity_pp(sf_ctx* c) {
 	const sf_params_t* p = sf_params();
 	const unsigned int N = c->N;
