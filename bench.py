@@ -7,9 +7,14 @@ one O(N^2) force evaluation (20 flops per interaction), kick, per-particle
 energies and FP64 reductions.  value = N^2 * steps / elapsed, in
 Ginteractions/s.
 
-  python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu-baseline]
+  python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu-baseline] [--no-extra]
 
-Prints one JSON line.  Only one GPU is used; --gpus above 1 exits.
+Prints one JSON line.  Only one GPU is used; --gpus above 1 exits.  The
+other two parts of the metric are measured after the timed PP steps and
+reported under "extra": the Barnes-Hut step at N = 4194304 (theta = 0.5)
+with a PP force evaluation of the same state and both force errors against
+the FP64 direct sum, and the adjoint KDK's backward/forward time ratio at
+N = 16384; plus the PP + SPH gas-ball run (N = 65536, 50 steps).
 """
 from __future__ import annotations
 
@@ -92,12 +97,156 @@ def cpu_baseline(budget_s: float = 12.0) -> dict:
                       f"-fopenmp -Og -std=c99 build of the reference sources, OMP_NUM_THREADS={threads})"}
 
 
+def rel_err_pcts(acc, ref):
+    e = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    return {"p50": float(np.percentile(e, 50)), "p90": float(np.percentile(e, 90)),
+            "p99": float(np.percentile(e, 99)), "max": float(e.max())}
+
+
+def bench_bh(steps: int = 3) -> dict:
+    """Barnes-Hut (SF_TREE_LBVH, theta = 0.5) KDK step at N = 4194304 against
+    one fast PP force evaluation of the same state; force errors of both
+    against the FP64 direct sum on every 1024th particle."""
+    N, theta, leaf = 4194304, 0.5, 8
+    common = dict(logging=0, epsilon=EPS, dt=DT, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL,
+                  mem_mode=L.SF_MEM_RESIDENT)
+    s = Simulation(N, enable_tree=1, tree_mode=L.SF_TREE_LBVH, theta=theta, leaf_size=leaf, **common)
+    lib = s.lib
+    s.ic_plummer(seed=21, first=0, count=N)
+    s.solve()
+    s.step()
+    hip_sync()
+    lib.sf_prof_enable(1)
+    lib.sf_prof_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step()
+    hip_sync()
+    step_ms = (time.perf_counter() - t0) / steps * 1e3
+    b_ms, b_n = kernel_ms(lib, "lbvh_build")
+    w_ms, w_n = kernel_ms(lib, "lbvh_walk")
+    lib.sf_prof_enable(0)
+    ex = s.lbvh_export()
+    s.to_host()
+    pos = s.pos.copy()
+    mas = s.mas.copy()
+    idx = np.arange(0, N, 1024, dtype=np.uint32)
+    ref, _ = s.direct_f64(idx)
+    bh_err = rel_err_pcts(s.acc[idx].astype(np.float64), ref)
+    s.close()
+    p = Simulation(N, enable_tree=0, **common)
+    p.pos[:] = pos
+    p.mas[:] = mas
+    p.to_device()
+    hip_sync()
+    t0 = time.perf_counter()
+    p.solve_gravity_pp()
+    hip_sync()
+    pp_ms = (time.perf_counter() - t0) * 1e3
+    p.to_host()
+    pp_err = rel_err_pcts(p.acc[idx].astype(np.float64), ref)
+    p.close()
+    walk_ms = w_ms / max(w_n, 1)
+    inter = float(ex["cells"] + ex["parts"])
+    return {"workload": f"Plummer sphere N={N} (a=1, M=1, seed 21), theta={theta}, leaf_size={leaf}, "
+                        f"eps={EPS}, dt={DT}, KDK step in the fast policy, state resident",
+            "N": N, "theta": theta, "step_ms": step_ms, "build_ms": b_ms / max(b_n, 1), "walk_ms": walk_ms,
+            "interactions_per_particle": inter / N, "walk_interactions_per_s": inter / (walk_ms / 1e3),
+            "force_err_vs_fp64": bh_err, "pp_force_ms": pp_ms, "pp_force_err_vs_fp64": pp_err,
+            "pp_force_over_bh_step": pp_ms / step_ms, "error_sample": "every 1024th particle (4096)"}
+
+
+def bench_adjoint(reps: int = 3) -> dict:
+    """Adjoint KDK on two Plummer spheres: N = 16384 (2 x 8192, a = 0.5,
+    centres (+-1.5, 0, 0), approach speed 0.3), eps = 0.01, dt = 1e-3, 200
+    steps, checkpoints every 20; loss = mean((x_N - x_target)^2), target run
+    from ICs perturbed by N(0, 1e-3).  Reports forward and backward times."""
+    import torch
+    from starflood_amd import adjoint as A
+    N, K, k = 16384, 200, 20
+    s = Simulation(N, enable_tree=0, logging=0, epsilon=1e-2, dt=1e-3)
+    s.ic_plummer(seed=31, first=0, count=N // 2, a=0.5, M=0.5, centre=(-1.5, 0, 0), bulk_v=(0.15, 0, 0))
+    s.ic_plummer(seed=32, first=N // 2, count=N // 2, a=0.5, M=0.5, centre=(1.5, 0, 0), bulk_v=(-0.15, 0, 0))
+    x0, v0, m = s.pos.copy(), s.vel.copy(), s.mas.copy()
+    s.close()
+    L.set_params(epsilon=1e-2, dt=1e-3, G=1.0, acc_mode=L.SF_ACC_PHYSICAL)
+    dev = torch.device("cuda")
+    tm = torch.from_numpy(m).to(dev)
+    rng = np.random.default_rng(33)
+    with torch.no_grad():
+        xt, _ = A.kdk(torch.from_numpy(x0 + rng.normal(scale=1e-3, size=x0.shape).astype(np.float32)).to(dev),
+                      torch.from_numpy(v0).to(dev), tm, K, k)
+    fwd, bwd = [], []
+    for _ in range(reps + 1):
+        tx = torch.from_numpy(x0).to(dev).requires_grad_(True)
+        tv = torch.from_numpy(v0).to(dev).requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xN, vN = A.kdk(tx, tv, tm, K, k)
+        loss = ((xN - xt) ** 2).mean()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fwd.append(t1 - t0)
+        bwd.append(t2 - t1)
+    f, b = float(np.median(fwd[1:])), float(np.median(bwd[1:]))
+    return {"workload": "two Plummer spheres N=16384, eps=0.01, dt=1e-3, 200 KDK steps, checkpoint every 20, "
+                        "MSE loss on final positions",
+            "forward_ms": f * 1e3, "backward_ms": b * 1e3, "backward_over_forward": b / f,
+            "loss": float(loss.item()), "grad_norm_x0": float(tx.grad.norm().item()),
+            "grad_norm_v0": float(tv.grad.norm().item())}
+
+
+def bench_sph(steps: int = 50) -> dict:
+    """PP gravity + cubic-spline SPH: N = 65536 gas particles uniform in the
+    unit ball (seeded), v = 0, m = 1/N, gamma = 5/3, u = 0.05, h = 0.05,
+    eps = 0.01, dt = 5e-4, 50 KDK steps; energy E = T + W + U and momentum
+    drift."""
+    N = 65536
+    s = Simulation(N, enable_tree=0, enable_sph=1, sph_model=L.SF_SPH_CUBIC_IDEAL, sph_gamma=5.0 / 3.0, logging=0,
+                   epsilon=1e-2, dt=5e-4, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL,
+                   mem_mode=L.SF_MEM_RESIDENT)
+    lib = s.lib
+    s.ic_uniform_ball(seed=41, radius=1.0, total_mass=1.0, v_hubble=0.0)
+    s.rad[:] = 0.05
+    s.vel[:] = 0.0
+    s.to_device()
+    s.set_u(np.full(N, 0.05, np.float32))
+    s.solve()
+    e0 = s.stats()
+    hip_sync()
+    lib.sf_prof_enable(1)
+    lib.sf_prof_reset()
+    t0 = time.perf_counter()
+    for n in range(steps):
+        s.step_number = n
+        s.step()
+    hip_sync()
+    step_ms = (time.perf_counter() - t0) / steps * 1e3
+    rho_ms, rho_n = kernel_ms(lib, "sph_rho_cubic")
+    frc_ms, frc_n = kernel_ms(lib, "sph_force_cubic")
+    pp_ms, pp_n = kernel_ms(lib, "pp_fast")
+    lib.sf_prof_enable(0)
+    e1 = s.stats()
+    s.close()
+    E0 = e0["T"] + e0["W"] + e0["U"]
+    E1 = e1["T"] + e1["W"] + e1["U"]
+    return {"workload": "PP gravity + PP cubic-spline SPH, uniform ball N=65536, gamma=5/3, u=0.05, h=0.05, "
+                        "eps=0.01, dt=5e-4, 50 KDK steps, fast policy, state resident",
+            "step_ms": step_ms, "sph_density_ms": rho_ms / max(rho_n, 1), "sph_force_ms": frc_ms / max(frc_n, 1),
+            "pp_force_ms": pp_ms / max(pp_n, 1), "energy_rel_drift": abs(E1 - E0) / abs(E0),
+            "momentum_abs": float(np.abs(e1["P"]).max()), "E0": E0, "E1": E1}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the BH, adjoint and SPH measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design "
@@ -151,6 +300,8 @@ def main():
                      "flops_per_launch": FLOPS_PER_INTERACTION * inter},
         "energy_check": {"E": st["E"], "T": st["T"], "W": st["W"]},
     }
+    if not args.no_extra:
+        out["extra"] = {"bh_4m": bench_bh(), "adjoint_16k": bench_adjoint(), "sph_65k": bench_sph()}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
