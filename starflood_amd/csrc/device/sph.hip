@@ -31,6 +31,7 @@
 namespace {
 
 constexpr unsigned kBlock = 256;
+constexpr unsigned kTj = 128;   // j per unrolled tile of the cubic-model passes
 
 // the reference's constants (src/common.h:18, :29), rounded to float as
 // (real)INV_SQRT_PI and (real)INV_POW_PI_3_2 are; the second one's value
@@ -137,24 +138,31 @@ __global__ __launch_bounds__(kBlock) void k_force_ref(const float* __restrict__ 
 }
 
 // ---------------------------------------------------------------- cubic model
-// Packed per-particle records: p0 = (x, y, z, m), p1 = (v_pred, P/rho^2),
-// p2 = (h, 1/h, h^2, u_pred).  v_pred = v + pred_dt acc and u_pred =
-// u + pred_dt du/dt predict the half-kicked state to the end of the step
-// (pred_dt = 0 uses the state as it is).
+// Packed per-particle records (n_pad entries, zero past N):
+//   q0 = (x, y, z, h^2), q1 = (m, 1/h, u_pred, P/rho^2), q2 = (v_pred, 0).
+// v_pred = v + pred_dt acc and u_pred = u + pred_dt du/dt predict the
+// half-kicked state to the end of the step (pred_dt = 0 takes it as it is).
 __global__ __launch_bounds__(kBlock) void k_pack_cubic(const float* __restrict__ pos, const float* __restrict__ vel,
                                                        const float* __restrict__ acc, const float* __restrict__ mas,
                                                        const float* __restrict__ rad, const float* __restrict__ u,
-                                                       const float* __restrict__ dudt, unsigned N, float pred_dt,
-                                                       float4* __restrict__ p0, float4* __restrict__ p1,
-                                                       float4* __restrict__ p2) {
+                                                       const float* __restrict__ dudt, unsigned N, unsigned n_pad,
+                                                       float pred_dt, float4* __restrict__ q0,
+                                                       float4* __restrict__ q1, float4* __restrict__ q2) {
 	const unsigned i = blockIdx.x * kBlock + threadIdx.x;
-	if (i >= N) return;
-	p0[i] = make_float4(pos[3u * i], pos[3u * i + 1u], pos[3u * i + 2u], mas[i]);
-	p1[i] = make_float4(__builtin_fmaf(pred_dt, acc[3u * i], vel[3u * i]),
+	if (i >= n_pad) return;
+	if (i >= N) {
+		const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		q0[i] = z;
+		q1[i] = z;
+		q2[i] = z;
+		return;
+	}
+	const float h = rad[i];
+	q0[i] = make_float4(pos[3u * i], pos[3u * i + 1u], pos[3u * i + 2u], h * h);
+	q1[i] = make_float4(mas[i], 1.0f / h, __builtin_fmaf(pred_dt, dudt[i], u[i]), 0.0f);
+	q2[i] = make_float4(__builtin_fmaf(pred_dt, acc[3u * i], vel[3u * i]),
 	                    __builtin_fmaf(pred_dt, acc[3u * i + 1u], vel[3u * i + 1u]),
 	                    __builtin_fmaf(pred_dt, acc[3u * i + 2u], vel[3u * i + 2u]), 0.0f);
-	const float h = rad[i];
-	p2[i] = make_float4(h, 1.0f / h, h * h, __builtin_fmaf(pred_dt, dudt[i], u[i]));
 }
 
 // W(r, h) and dW/dr(r, h) of the cubic spline for 0 <= q = r/h
@@ -171,70 +179,113 @@ __device__ __forceinline__ float cubic_dw(float q, float inv_h) {
 	return q <= 0.5f ? k * inner : (q <= 1.0f ? k * -6.0f * t * t : 0.0f);
 }
 
-// rho_i, P_i = (gamma - 1) rho_i u_i, and P_i / rho_i^2 into p1.w
-__global__ __launch_bounds__(kBlock) void k_rho_cubic(const float4* __restrict__ p0, float4* __restrict__ p1,
-                                                      const float4* __restrict__ p2, unsigned N, float gm1,
-                                                      float* __restrict__ rho, float* __restrict__ prs) {
-	const unsigned i = blockIdx.x * kBlock + threadIdx.x;
-	const unsigned ii = i < N ? i : N - 1u;
-	const float4 a = p0[ii];
-	const float4 ha = p2[ii];
+// Both passes split j into slices over the grid (slice = blockIdx % slices,
+// as the fast PP kernel does) so that a few thousand particles still give
+// every SIMD several wavefronts; the neighbour test is 8 VALU operations per
+// pair and the kernel arithmetic runs only for lanes with a neighbour.
+__global__ __launch_bounds__(kBlock) void k_rho_cubic(const float4* __restrict__ q0, const float4* __restrict__ q1,
+                                                      unsigned n_pad, unsigned slices, unsigned slice_len,
+                                                      float* __restrict__ part) {
+	const unsigned b = blockIdx.x;
+	const unsigned sl = b % slices;
+	const unsigned i = (b / slices) * kBlock + threadIdx.x;
+	const float4 a = q0[i];
+	const float inv_hi = q1[i].y;
 	float s = 0.0f;
-	for (unsigned j = 0u; j < N; ++j) {
-		const float4 b = p0[j];
-		const float4 hb = p2[j];
-		const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-		const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-		if (r2 < fmaxf(ha.z, hb.z)) {
-			const float r = __builtin_sqrtf(r2);
-			const float w = cubic_w(r * ha.y, ha.y) + cubic_w(r * hb.y, hb.y);
-			s = __builtin_fmaf(0.5f * b.w, w, s);
+	const unsigned j0 = sl * slice_len;
+	const unsigned j1 = min(j0 + slice_len, n_pad);
+	for (unsigned jt = j0; jt < j1; jt += kTj) {
+#pragma unroll 8
+		for (unsigned j = jt; j < jt + kTj; ++j) {
+			const float4 b4 = q0[j];
+			const float dx = a.x - b4.x, dy = a.y - b4.y, dz = a.z - b4.z;
+			const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+			if (r2 < fmaxf(a.w, b4.w)) {
+				const float4 c = q1[j];
+				const float r = __builtin_sqrtf(r2);
+				const float w = cubic_w(r * inv_hi, inv_hi) + cubic_w(r * c.y, c.y);
+				s = __builtin_fmaf(0.5f * c.x, w, s);
+			}
 		}
 	}
-	if (i < N) {
-		const float p = gm1 * s * ha.w;
-		rho[i] = s;
-		prs[i] = p;
-		float4 v = p1[i];
-		v.w = s > 0.0f ? p / (s * s) : 0.0f;
-		p1[i] = v;
-	}
+	part[(size_t)sl * n_pad + i] = s;
 }
 
-__global__ __launch_bounds__(kBlock) void k_force_cubic(const float4* __restrict__ p0, const float4* __restrict__ p1,
-                                                        const float4* __restrict__ p2, unsigned N, int acc_phys,
-                                                        float* __restrict__ acc, float* __restrict__ dudt) {
+// rho = sum of the slice partials in fixed order; P = (gamma - 1) rho u;
+// P / rho^2 into q1.w for the force pass
+__global__ __launch_bounds__(kBlock) void k_rho_cubic_combine(const float* __restrict__ part, unsigned n_pad,
+                                                              unsigned slices, unsigned N, float gm1,
+                                                              float4* __restrict__ q1, float* __restrict__ rho,
+                                                              float* __restrict__ prs) {
 	const unsigned i = blockIdx.x * kBlock + threadIdx.x;
-	const unsigned ii = i < N ? i : N - 1u;
-	const float4 a = p0[ii];
-	const float4 va = p1[ii];
-	const float4 ha = p2[ii];
+	if (i >= N) return;
+	double s = 0.0;
+	for (unsigned k = 0; k < slices; ++k) s += (double)part[(size_t)k * n_pad + i];
+	const float r = (float)s;
+	float4 c = q1[i];
+	const float p = gm1 * r * c.z;
+	c.w = r > 0.0f ? p / (r * r) : 0.0f;
+	q1[i] = c;
+	rho[i] = r;
+	prs[i] = p;
+}
+
+__global__ __launch_bounds__(kBlock) void k_force_cubic(const float4* __restrict__ q0, const float4* __restrict__ q1,
+                                                        const float4* __restrict__ q2, unsigned n_pad, unsigned slices,
+                                                        unsigned slice_len, float4* __restrict__ part) {
+	const unsigned b = blockIdx.x;
+	const unsigned sl = b % slices;
+	const unsigned i = (b / slices) * kBlock + threadIdx.x;
+	const float4 a = q0[i];
+	const float4 ca = q1[i];
+	const float4 va = q2[i];
 	float ax = 0.0f, ay = 0.0f, az = 0.0f, du = 0.0f;
-	for (unsigned j = 0u; j < N; ++j) {
-		const float4 b = p0[j];
-		const float4 hb = p2[j];
-		const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-		const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-		if (r2 < fmaxf(ha.z, hb.z) && r2 > 0.0f) {
-			const float4 vb = p1[j];
-			const float inv_r = __builtin_amdgcn_rsqf(r2);
-			const float r = r2 * inv_r;
-			const float dw = cubic_dw(r * ha.y, ha.y) + cubic_dw(r * hb.y, hb.y);
-			const float f = 0.5f * b.w * (va.w + vb.w) * dw * inv_r;   // grad_i W = dW/dr d / r
-			ax = __builtin_fmaf(-f, dx, ax);
-			ay = __builtin_fmaf(-f, dy, ay);
-			az = __builtin_fmaf(-f, dz, az);
-			const float vd = __builtin_fmaf(va.z - vb.z, dz, __builtin_fmaf(va.y - vb.y, dy, (va.x - vb.x) * dx));
-			du = __builtin_fmaf(0.5f * f, vd, du);
+	const unsigned j0 = sl * slice_len;
+	const unsigned j1 = min(j0 + slice_len, n_pad);
+	for (unsigned jt = j0; jt < j1; jt += kTj) {
+#pragma unroll 8
+		for (unsigned j = jt; j < jt + kTj; ++j) {
+			const float4 b4 = q0[j];
+			const float dx = a.x - b4.x, dy = a.y - b4.y, dz = a.z - b4.z;
+			const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+			if (r2 < fmaxf(a.w, b4.w) && r2 > 0.0f) {
+				const float4 cb = q1[j];
+				const float4 vb = q2[j];
+				const float inv_r = __builtin_amdgcn_rsqf(r2);
+				const float r = r2 * inv_r;
+				const float dw = cubic_dw(r * ca.y, ca.y) + cubic_dw(r * cb.y, cb.y);
+				const float f = 0.5f * cb.x * (ca.w + cb.w) * dw * inv_r;   // grad_i W = dW/dr d / r
+				ax = __builtin_fmaf(-f, dx, ax);
+				ay = __builtin_fmaf(-f, dy, ay);
+				az = __builtin_fmaf(-f, dz, az);
+				const float vd =
+				    __builtin_fmaf(va.z - vb.z, dz, __builtin_fmaf(va.y - vb.y, dy, (va.x - vb.x) * dx));
+				du = __builtin_fmaf(0.5f * f, vd, du);
+			}
 		}
 	}
-	if (i < N) {
-		const float sc = acc_phys ? 1.0f : a.w;
-		acc[3u * i + 0u] += sc * ax;
-		acc[3u * i + 1u] += sc * ay;
-		acc[3u * i + 2u] += sc * az;
-		dudt[i] = du;
+	part[(size_t)sl * n_pad + i] = make_float4(ax, ay, az, du);
+}
+
+__global__ __launch_bounds__(kBlock) void k_force_cubic_combine(const float4* __restrict__ part, unsigned n_pad,
+                                                                unsigned slices, unsigned N, int acc_phys,
+                                                                const float4* __restrict__ q1,
+                                                                float* __restrict__ acc, float* __restrict__ dudt) {
+	const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= N) return;
+	double ax = 0.0, ay = 0.0, az = 0.0, du = 0.0;
+	for (unsigned k = 0; k < slices; ++k) {
+		const float4 p = part[(size_t)k * n_pad + i];
+		ax += (double)p.x;
+		ay += (double)p.y;
+		az += (double)p.z;
+		du += (double)p.w;
 	}
+	const double sc = acc_phys ? 1.0 : (double)q1[i].x;
+	acc[3u * i + 0u] += (float)(sc * ax);
+	acc[3u * i + 1u] += (float)(sc * ay);
+	acc[3u * i + 2u] += (float)(sc * az);
+	dudt[i] = (float)du;
 }
 
 __global__ __launch_bounds__(kBlock) void k_u_kick(float* __restrict__ u, const float* __restrict__ dudt, unsigned N,
@@ -249,7 +300,7 @@ inline dim3 grid(unsigned n) { return dim3((n + kBlock - 1u) / kBlock); }
 
 extern "C" {
 
-size_t sfd_sph_tmp_floats(unsigned N) { return 12u * (size_t)N + 16u; }
+size_t sfd_sph_tmp_floats(unsigned n_pad) { return 12u * (size_t)n_pad + 16u; }
 
 int sfd_sph_ref(const float* pos, const float* mas, const float* rad, unsigned N, int kahan, float* rho, float* prs,
                 float* acc, float* tmp, hipStream_t s) {
@@ -268,25 +319,38 @@ int sfd_sph_ref(const float* pos, const float* mas, const float* rad, unsigned N
 }
 
 int sfd_sph_cubic_pack(const float* pos, const float* vel, const float* acc, const float* mas, const float* rad,
-                       const float* u, const float* dudt, unsigned N, float pred_dt, float* tmp, hipStream_t s) {
+                       const float* u, const float* dudt, unsigned N, unsigned n_pad, float pred_dt, float* tmp,
+                       hipStream_t s) {
 	if (N == 0) return 0;
-	float4* p0 = reinterpret_cast<float4*>(tmp);
-	hipLaunchKernelGGL(k_pack_cubic, grid(N), dim3(kBlock), 0, s, pos, vel, acc, mas, rad, u, dudt, N, pred_dt, p0,
-	                   p0 + N, p0 + 2u * (size_t)N);
+	float4* q0 = reinterpret_cast<float4*>(tmp);
+	hipLaunchKernelGGL(k_pack_cubic, grid(n_pad), dim3(kBlock), 0, s, pos, vel, acc, mas, rad, u, dudt, N, n_pad,
+	                   pred_dt, q0, q0 + n_pad, q0 + 2u * (size_t)n_pad);
 	return sf_hip_check(hipGetLastError(), "solve_sph", "launch");
 }
 
-int sfd_sph_cubic(unsigned N, float gamma, int acc_phys, float* rho, float* prs, float* acc, float* dudt, float* tmp,
-                  hipStream_t s) {
+// part: slices x n_pad float4 of scratch
+int sfd_sph_cubic(unsigned N, unsigned n_pad, float gamma, int acc_phys, float* rho, float* prs, float* acc,
+                  float* dudt, float* tmp, float* part, unsigned slices, hipStream_t s) {
 	if (N == 0) return 0;
-	float4* p0 = reinterpret_cast<float4*>(tmp);
-	float4* p1 = p0 + N;
-	float4* p2 = p0 + 2u * (size_t)N;
+	if (n_pad % kTj != 0 || n_pad % kBlock != 0 || slices == 0) return -1;
+	float4* q0 = reinterpret_cast<float4*>(tmp);
+	float4* q1 = q0 + n_pad;
+	float4* q2 = q0 + 2u * (size_t)n_pad;
+	const unsigned tiles = n_pad / kTj;
+	if (slices > tiles) slices = tiles;
+	const unsigned slice_len = (tiles + slices - 1u) / slices * kTj;
+	slices = (n_pad + slice_len - 1u) / slice_len;
+	const dim3 g((n_pad / kBlock) * slices);
 	const int tok = sf_prof_begin("sph_rho_cubic", s);
-	hipLaunchKernelGGL(k_rho_cubic, grid(N), dim3(kBlock), 0, s, p0, p1, p2, N, gamma - 1.0f, rho, prs);
+	hipLaunchKernelGGL(k_rho_cubic, g, dim3(kBlock), 0, s, q0, q1, n_pad, slices, slice_len, part);
+	hipLaunchKernelGGL(k_rho_cubic_combine, grid(N), dim3(kBlock), 0, s, part, n_pad, slices, N, gamma - 1.0f, q1,
+	                   rho, prs);
 	sf_prof_end(tok, s);
 	const int tok2 = sf_prof_begin("sph_force_cubic", s);
-	hipLaunchKernelGGL(k_force_cubic, grid(N), dim3(kBlock), 0, s, p0, p1, p2, N, acc_phys, acc, dudt);
+	float4* p4 = reinterpret_cast<float4*>(part);
+	hipLaunchKernelGGL(k_force_cubic, g, dim3(kBlock), 0, s, q0, q1, q2, n_pad, slices, slice_len, p4);
+	hipLaunchKernelGGL(k_force_cubic_combine, grid(N), dim3(kBlock), 0, s, p4, n_pad, slices, N, acc_phys, q1, acc,
+	                   dudt);
 	sf_prof_end(tok2, s);
 	return sf_hip_check(hipGetLastError(), "solve_sph", "launch");
 }

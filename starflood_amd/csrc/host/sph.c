@@ -11,17 +11,18 @@
 #include "sf_host.h"
 
 /* kernels in device/sph.hip */
-size_t sfd_sph_tmp_floats(unsigned int N);
+size_t sfd_sph_tmp_floats(unsigned int n_pad);
 int sfd_sph_ref(const float* pos, const float* mas, const float* rad, unsigned int N, int kahan, float* rho, float* prs,
                 float* acc, float* tmp, hipStream_t s);
 int sfd_sph_cubic_pack(const float* pos, const float* vel, const float* acc, const float* mas, const float* rad,
-                       const float* u, const float* dudt, unsigned int N, float pred_dt, float* tmp, hipStream_t s);
-int sfd_sph_cubic(unsigned int N, float gamma, int acc_phys, float* rho, float* prs, float* acc, float* dudt,
-                  float* tmp, hipStream_t s);
+                       const float* u, const float* dudt, unsigned int N, unsigned int n_pad, float pred_dt, float* tmp,
+                       hipStream_t s);
+int sfd_sph_cubic(unsigned int N, unsigned int n_pad, float gamma, int acc_phys, float* rho, float* prs, float* acc,
+                  float* dudt, float* tmp, float* part, unsigned int slices, hipStream_t s);
 
 static int ensure_tmp(sf_ctx* c) {
 	if (c->d_sph_tmp) return 0;
-	SF_HIP(hipMalloc((void**)&c->d_sph_tmp, sfd_sph_tmp_floats(c->N) * sizeof(float)), "solve_sph");
+	SF_HIP(hipMalloc((void**)&c->d_sph_tmp, sfd_sph_tmp_floats(c->n_pad) * sizeof(float)), "solve_sph");
 	return 0;
 }
 
@@ -32,7 +33,8 @@ int sf_sph_prepare(sf_ctx* c, float pred_dt) {
 	const unsigned int N = c->N;
 	return sfd_sph_cubic_pack(sf_blk(c->d_mem, N, SF_E_POS), sf_blk(c->d_mem, N, SF_E_VEL),
 	                          sf_blk(c->d_mem, N, SF_E_ACC), sf_blk(c->d_mem, N, SF_E_MAS),
-	                          sf_blk(c->d_mem, N, SF_E_RAD), c->d_u, c->d_dudt, N, pred_dt, c->d_sph_tmp, c->stream);
+	                          sf_blk(c->d_mem, N, SF_E_RAD), c->d_u, c->d_dudt, N, c->n_pad, pred_dt, c->d_sph_tmp,
+	                          c->stream);
 }
 
 int sf_solve_sph(sf_ctx* c) {
@@ -43,9 +45,11 @@ int sf_solve_sph(sf_ctx* c) {
 	float* prs = sf_blk(c->d_mem, N, SF_E_PRS);
 	float* acc = sf_blk(c->d_mem, N, SF_E_ACC);
 	if (p->sph_model == SF_SPH_CUBIC_IDEAL) {
-		if (sf_ctx_ensure_u(c)) return -1;
-		return sfd_sph_cubic(N, (float)p->sph_gamma, p->acc_mode == SF_ACC_PHYSICAL, rho, prs, acc, c->d_dudt,
-		                     c->d_sph_tmp, c->stream);
+		/* slice partials reuse the PP kernel's scratch (same stream, after it) */
+		const unsigned int slices = sf_pp_slice_count(c->n_pad);
+		if (sf_ctx_ensure_u(c) || sf_ctx_reserve_part(c, slices)) return -1;
+		return sfd_sph_cubic(N, c->n_pad, (float)p->sph_gamma, p->acc_mode == SF_ACC_PHYSICAL, rho, prs, acc,
+		                     c->d_dudt, c->d_sph_tmp, c->d_part, slices, c->stream);
 	}
 	const int tok = sf_prof_begin("sph_ref", c->stream);
 	const int rc = sfd_sph_ref(sf_blk(c->d_mem, N, SF_E_POS), sf_blk(c->d_mem, N, SF_E_MAS),
