@@ -103,7 +103,7 @@ def rel_err_pcts(acc, ref):
             "p99": float(np.percentile(e, 99)), "max": float(e.max())}
 
 
-def bench_bh(steps: int = 3) -> dict:
+def bench_bh(steps: int = 3, pp_compare: bool = True) -> dict:
     """Barnes-Hut (SF_TREE_LBVH, theta = 0.5) KDK step at N = 4194304 against
     one fast PP force evaluation of the same state; force errors of both
     against the FP64 direct sum on every 1024th particle."""
@@ -134,6 +134,15 @@ def bench_bh(steps: int = 3) -> dict:
     ref, _ = s.direct_f64(idx)
     bh_err = rel_err_pcts(s.acc[idx].astype(np.float64), ref)
     s.close()
+    walk_ms = w_ms / max(w_n, 1)
+    inter = float(ex["cells"] + ex["parts"])
+    out = {"workload": f"Plummer sphere N={N} (a=1, M=1, seed 21), theta={theta}, leaf_size={leaf}, "
+                       f"eps={EPS}, dt={DT}, KDK step in the fast policy, state resident",
+           "N": N, "theta": theta, "step_ms": step_ms, "build_ms": b_ms / max(b_n, 1), "walk_ms": walk_ms,
+           "interactions_per_particle": inter / N, "walk_interactions_per_s": inter / (walk_ms / 1e3),
+           "force_err_vs_fp64": bh_err, "error_sample": "every 1024th particle (4096)"}
+    if not pp_compare:
+        return out
     p = Simulation(N, enable_tree=0, **common)
     p.pos[:] = pos
     p.mas[:] = mas
@@ -146,14 +155,20 @@ def bench_bh(steps: int = 3) -> dict:
     p.to_host()
     pp_err = rel_err_pcts(p.acc[idx].astype(np.float64), ref)
     p.close()
-    walk_ms = w_ms / max(w_n, 1)
-    inter = float(ex["cells"] + ex["parts"])
-    return {"workload": f"Plummer sphere N={N} (a=1, M=1, seed 21), theta={theta}, leaf_size={leaf}, "
-                        f"eps={EPS}, dt={DT}, KDK step in the fast policy, state resident",
-            "N": N, "theta": theta, "step_ms": step_ms, "build_ms": b_ms / max(b_n, 1), "walk_ms": walk_ms,
-            "interactions_per_particle": inter / N, "walk_interactions_per_s": inter / (walk_ms / 1e3),
-            "force_err_vs_fp64": bh_err, "pp_force_ms": pp_ms, "pp_force_err_vs_fp64": pp_err,
-            "pp_force_over_bh_step": pp_ms / step_ms, "error_sample": "every 1024th particle (4096)"}
+    out.update({"pp_force_ms": pp_ms, "pp_force_err_vs_fp64": pp_err, "pp_force_over_bh_step": pp_ms / step_ms})
+    return out
+
+
+def bench_copy(mib: int = 1024) -> dict:
+    """A plain device copy of known size (torch), the calibration run for the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    a = torch.ones(mib * 262144, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4, "launches": 3}
 
 
 def bench_adjoint(reps: int = 3) -> dict:
@@ -247,11 +262,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the BH, adjoint and SPH measurements")
+    ap.add_argument("--only", choices=["bh", "adjoint", "sph", "copy"],
+                    help="run one secondary workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design "
                                    "(see DESIGN.md), not a run", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.only:
+        fn = {"bh": lambda: bench_bh(steps=args.steps, pp_compare=False), "adjoint": lambda: bench_adjoint(reps=1),
+              "sph": lambda: bench_sph(steps=args.steps), "copy": bench_copy}[args.only]
+        print(json.dumps(fn()))
+        return
 
     sim = Simulation(N_PP, enable_tree=0, logging=0, verbose=0, epsilon=EPS, dt=DT,
                      precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL, mem_mode=L.SF_MEM_RESIDENT)
