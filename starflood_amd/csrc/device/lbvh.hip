@@ -135,8 +135,60 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const unsigned long long*
 	hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of the digit-major histogram (256 * ntiles entries) in one
-// block: each thread scans a contiguous chunk, then the chunk totals.
+// Exclusive scan of the digit-major histogram (256 * ntiles entries) in three
+// passes: per-chunk totals (k_scan_u32_sums), an exclusive scan of the chunk
+// totals in one block (k_radix_scan), and the scan of each chunk from its
+// offset (k_scan_u32_down).  A chunk is 4096 entries, 16 per thread.
+constexpr unsigned kScanU32Per = 16;
+constexpr unsigned kScanU32Chunk = kBlock * kScanU32Per;
+
+__global__ __launch_bounds__(kBlock) void k_scan_u32_sums(const unsigned* __restrict__ data, unsigned n,
+                                                          unsigned* __restrict__ sums) {
+	__shared__ unsigned ws[kBlock / 64];
+	const unsigned base = blockIdx.x * kScanU32Chunk + threadIdx.x * kScanU32Per;
+	unsigned s = 0;
+#pragma unroll
+	for (unsigned k = 0; k < kScanU32Per; ++k) s += base + k < n ? data[base + k] : 0u;
+	for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+	if ((threadIdx.x & 63u) == 0) ws[threadIdx.x >> 6] = s;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		unsigned t = 0;
+		for (unsigned q = 0; q < kBlock / 64; ++q) t += ws[q];
+		sums[blockIdx.x] = t;
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_u32_down(unsigned* __restrict__ data, unsigned n,
+                                                          const unsigned* __restrict__ offs) {
+	__shared__ unsigned ws[kBlock / 64];
+	const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+	const unsigned base = blockIdx.x * kScanU32Chunk + threadIdx.x * kScanU32Per;
+	unsigned v[kScanU32Per];
+	unsigned s = 0;
+#pragma unroll
+	for (unsigned k = 0; k < kScanU32Per; ++k) {
+		v[k] = base + k < n ? data[base + k] : 0u;
+		s += v[k];
+	}
+	unsigned inc = s;
+	for (int o = 1; o < 64; o <<= 1) {
+		const unsigned t = __shfl_up(inc, o);
+		if ((int)lane >= o) inc += t;
+	}
+	if (lane == 63u) ws[w] = inc;
+	__syncthreads();
+	unsigned run = offs[blockIdx.x] + inc - s;
+	for (unsigned q = 0; q < w; ++q) run += ws[q];
+#pragma unroll
+	for (unsigned k = 0; k < kScanU32Per; ++k) {
+		if (base + k < n) data[base + k] = run;
+		run += v[k];
+	}
+}
+
+// Exclusive scan of n entries in one block: each thread scans a contiguous
+// chunk, then the chunk totals (the top level of the histogram scan).
 __global__ __launch_bounds__(1024) void k_radix_scan(unsigned* __restrict__ hist, unsigned n) {
 	__shared__ unsigned tot[1024];
 	const unsigned chunk = (n + 1023) / 1024;
@@ -461,6 +513,11 @@ int sfd_lbvh_keys(const float* pos, unsigned N, unsigned n_sort, double* bbox_pa
 
 unsigned sfd_radix_tile(void) { return kTile; }
 
+unsigned sfd_radix_hist_entries(unsigned n_sort) {
+	const unsigned nh = 256u * (n_sort / kTile);
+	return nh + (nh + kScanU32Chunk - 1u) / kScanU32Chunk;
+}
+
 unsigned sfd_lbvh_scan_tiles(unsigned N) { return (N + kScanTile - 1) / kScanTile; }
 
 // Sorts n_sort (a multiple of the tile) pairs; the result ends in (key0,
@@ -468,6 +525,9 @@ unsigned sfd_lbvh_scan_tiles(unsigned N) { return (N + kScanTile - 1) / kScanTil
 int sfd_radix_sort(unsigned long long* key0, unsigned* val0, unsigned long long* key1, unsigned* val1,
                    unsigned n_sort, unsigned* hist, hipStream_t s) {
 	const unsigned ntiles = n_sort / kTile;
+	const unsigned nh = 256u * ntiles;
+	const unsigned nchunks = (nh + kScanU32Chunk - 1u) / kScanU32Chunk;
+	unsigned* chunk_sums = hist + nh;   // sfd_radix_hist_entries() leaves room for them
 	for (unsigned pass = 0; pass < 8; ++pass) {
 		const unsigned shift = 8u * pass;
 		unsigned long long* kin = (pass & 1) ? key1 : key0;
@@ -475,7 +535,9 @@ int sfd_radix_sort(unsigned long long* key0, unsigned* val0, unsigned long long*
 		unsigned long long* kout = (pass & 1) ? key0 : key1;
 		unsigned* vout = (pass & 1) ? val0 : val1;
 		hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, kin, shift, ntiles, hist);
-		hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, hist, 256u * ntiles);
+		hipLaunchKernelGGL(k_scan_u32_sums, dim3(nchunks), dim3(kBlock), 0, s, hist, nh, chunk_sums);
+		hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, chunk_sums, nchunks);
+		hipLaunchKernelGGL(k_scan_u32_down, dim3(nchunks), dim3(kBlock), 0, s, hist, nh, chunk_sums);
 		hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, shift, ntiles, hist, kout, vout);
 	}
 	return sf_hip_check(hipGetLastError(), "sfd_radix_sort", "launch");

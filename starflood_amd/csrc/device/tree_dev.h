@@ -35,6 +35,8 @@ int sfd_tree_walk_ref(const float* pos, const float* mas, unsigned int N, const 
    device, passed untyped through C --- */
 unsigned int sfd_lbvh_bbox_blocks(void);
 unsigned int sfd_radix_tile(void);
+/* u32 entries of the radix-sort histogram scratch for n_sort keys */
+unsigned int sfd_radix_hist_entries(unsigned int n_sort);
 /* bbox (FP64), then 63-bit Morton keys and identity values for n_sort
    (a multiple of the radix tile) entries; padding keys are all ones */
 int sfd_lbvh_keys(const float* pos, unsigned int N, unsigned int n_sort, double* bbox_part, double* box,

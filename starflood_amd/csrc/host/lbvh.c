@@ -73,7 +73,7 @@ static int ensure(sf_lbvh** st, unsigned int N) {
 	rc |= alloc((void**)&t->key1, n_sort * 8u);
 	rc |= alloc((void**)&t->val0, n_sort * 4u);
 	rc |= alloc((void**)&t->val1, n_sort * 4u);
-	rc |= alloc((void**)&t->hist, 256u * (n_sort / tile) * 4u);
+	rc |= alloc((void**)&t->hist, (size_t)sfd_radix_hist_entries((unsigned int)n_sort) * 4u);
 	rc |= alloc((void**)&t->bbox_part, sfd_lbvh_bbox_blocks() * 6u * 8u);
 	rc |= alloc((void**)&t->box, 4u * 8u);
 	rc |= alloc(&t->body, (size_t)N * 16u);
