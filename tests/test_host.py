@@ -1,0 +1,38 @@
+"""Host-side behaviour without a GPU: the benchmark's one-GPU contract and the
+argument checks of the device-array entry points (which return before any
+device call)."""
+import ctypes as C
+import json
+import subprocess
+import sys
+
+from conftest import REPO
+from starflood_amd import _lib as L
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, str(REPO / "bench.py"), "--gpus", "4"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert msg["n_gpus"] == 4 and "one GPU" in msg["error"]
+
+
+def test_adjoint_entry_points_validate_arguments():
+    lib = L.load()
+    tape = C.c_void_p()
+    # null arrays, N = 0 and ckpt_every = 0 are rejected before any device work
+    assert lib.sf_adjoint_forward(16, None, None, None, 10, 5, None, None, None, C.byref(tape)) == -1
+    assert "invalid arguments" in L.last_error()
+    assert tape.value is None
+    assert lib.sf_adjoint_backward(None, None, None, None, None, None) == -1
+    assert "null tape" in L.last_error()
+    assert lib.sf_force_vjp_dev(16, None, None, None, None, None) == -1
+    assert lib.sf_force_dev(0, None, None, None, None, None) == 0          # N = 0 is a no-op
+    lib.sf_tape_free(None)
+
+
+def test_lbvh_export_without_a_tree_fails_cleanly():
+    lib = L.load()
+    t = L.TreeT()
+    assert lib.sf_lbvh_export(C.byref(t), None, None, None, None, None) == -1
