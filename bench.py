@@ -41,6 +41,11 @@ DT = 1.0e-3
 SEED = 11
 PEAK_FP32_TFLOPS = 157.3
 FLOPS_PER_INTERACTION = 20
+# HBM bytes of one k_pp_fast dispatch at N = 262144 (32 j slices): rocprofv3
+# FETCH_SIZE 18324 KiB x 2.0 + WRITE_SIZE 262144 KiB x 1.0, the factors from a
+# 1 GiB copy read under the same counters (profiles/pp_fast_n262144.md).  The
+# writes are the slice partials; the kernel is VALU-bound.
+PP_FAST_TRAFFIC_BYTES = (18324 * 2 + 262144) * 1024
 
 
 def hip_sync():
@@ -317,7 +322,7 @@ def main():
                                "dt=1e-3, fast FP32 policy, state resident in HBM",
                    "N": N_PP, "epsilon": EPS, "dt": DT, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": PP_FAST_TRAFFIC_BYTES,
                      "kernel": "k_pp_fast", "kernel_ms": pp_avg_s * 1e3,
                      "flops_per_launch": FLOPS_PER_INTERACTION * inter},
         "energy_check": {"E": st["E"], "T": st["T"], "W": st["W"]},

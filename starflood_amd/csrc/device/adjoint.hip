@@ -23,6 +23,8 @@ namespace {
 
 constexpr unsigned kBlock = 256;
 constexpr unsigned kTile = 128;
+constexpr int kPrefetch = 4;   // j records per scalar-load group
+static_assert(kTile % kPrefetch == 0, "tiles hold whole prefetch groups");
 
 __device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
 	const float y = x - c;
@@ -42,31 +44,52 @@ __global__ __launch_bounds__(kBlock) void k_vjp(const float4* __restrict__ body,
 	const float4 bk = body[k];
 	const float4 nk = nu[k];
 	float Sx = 0.0f, Sy = 0.0f, Sz = 0.0f, Cx = 0.0f, Cy = 0.0f, Cz = 0.0f;
+	// one group of kPrefetch j records in flight, as in the fast PP kernel
 	const unsigned j0 = sl * slice_len;
 	const unsigned j1 = min(j0 + slice_len, n_pad);
+	float4 nbb[kPrefetch], nbn[kPrefetch];
+#pragma unroll
+	for (int u = 0; u < kPrefetch; ++u) {
+		nbb[u] = body[j0 + u];
+		nbn[u] = nu[j0 + u];
+	}
 	for (unsigned jt = j0; jt < j1; jt += kTile) {
 		float tx = 0.0f, ty = 0.0f, tz = 0.0f;
-#pragma unroll 4
-		for (unsigned j = jt; j < jt + kTile; ++j) {
-			const float4 bj = body[j];
-			const float4 nj = nu[j];
-			const float dx = bk.x - bj.x, dy = bk.y - bj.y, dz = bk.z - bj.z;
-			float r2 = __builtin_fmaf(dx, dx, eps2);
-			r2 = __builtin_fmaf(dy, dy, r2);
-			r2 = __builtin_fmaf(dz, dz, r2);
-			const float inv = __builtin_amdgcn_rsqf(r2);
-			const float inv2 = inv * inv;
-			const float inv3 = inv2 * inv;
-			const float i5_3 = inv3 * (3.0f * inv2);
-			// two rounded products: exactly zero when j == k
-			const float wx = bk.w * nj.x - bj.w * nk.x;
-			const float wy = bk.w * nj.y - bj.w * nk.y;
-			const float wz = bk.w * nj.z - bj.w * nk.z;
-			const float dw = __builtin_fmaf(dz, wz, __builtin_fmaf(dy, wy, dx * wx));
-			const float q = i5_3 * dw;
-			tx = __builtin_fmaf(-q, dx, __builtin_fmaf(inv3, wx, tx));
-			ty = __builtin_fmaf(-q, dy, __builtin_fmaf(inv3, wy, ty));
-			tz = __builtin_fmaf(-q, dz, __builtin_fmaf(inv3, wz, tz));
+		for (unsigned j = jt; j < jt + kTile; j += kPrefetch) {
+			float4 cbb[kPrefetch], cbn[kPrefetch];
+#pragma unroll
+			for (int u = 0; u < kPrefetch; ++u) {
+				cbb[u] = nbb[u];
+				cbn[u] = nbn[u];
+			}
+			const unsigned jn = (j + kPrefetch < j1) ? j + kPrefetch : j;   // in bounds
+#pragma unroll
+			for (int u = 0; u < kPrefetch; ++u) {
+				nbb[u] = body[jn + u];
+				nbn[u] = nu[jn + u];
+			}
+#pragma unroll
+			for (int u = 0; u < kPrefetch; ++u) {
+				const float4 bj = cbb[u];
+				const float4 nj = cbn[u];
+				const float dx = bk.x - bj.x, dy = bk.y - bj.y, dz = bk.z - bj.z;
+				float r2 = __builtin_fmaf(dx, dx, eps2);
+				r2 = __builtin_fmaf(dy, dy, r2);
+				r2 = __builtin_fmaf(dz, dz, r2);
+				const float inv = __builtin_amdgcn_rsqf(r2);
+				const float inv2 = inv * inv;
+				const float inv3 = inv2 * inv;
+				const float i5_3 = inv3 * (3.0f * inv2);
+				// two rounded products: exactly zero when j == k
+				const float wx = bk.w * nj.x - bj.w * nk.x;
+				const float wy = bk.w * nj.y - bj.w * nk.y;
+				const float wz = bk.w * nj.z - bj.w * nk.z;
+				const float dw = __builtin_fmaf(dz, wz, __builtin_fmaf(dy, wy, dx * wx));
+				const float q = i5_3 * dw;
+				tx = __builtin_fmaf(-q, dx, __builtin_fmaf(inv3, wx, tx));
+				ty = __builtin_fmaf(-q, dy, __builtin_fmaf(inv3, wy, ty));
+				tz = __builtin_fmaf(-q, dz, __builtin_fmaf(inv3, wz, tz));
+			}
 		}
 		kahan_add(Sx, Cx, tx);
 		kahan_add(Sy, Cy, ty);

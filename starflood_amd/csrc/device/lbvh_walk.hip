@@ -90,7 +90,9 @@ __device__ __forceinline__ void eval_list(Acc& a, float xi, float yi, float zi, 
 	fold(a);
 }
 
-__global__ __launch_bounds__(kBlock) void k_walk(const float4* __restrict__ body, const unsigned* __restrict__ order,
+// 5 wavefronts per SIMD: the register budget that keeps the walk spill-free
+// (88 VGPRs) while raising occupancy from 4 (measured 7% faster at 4M)
+__global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ body, const unsigned* __restrict__ order,
                                                  int N, const unsigned* __restrict__ group_start,
                                                  const float4* __restrict__ rec_a,
                                                  const int4* __restrict__ rec_b, float eps2, double G, int acc_phys,

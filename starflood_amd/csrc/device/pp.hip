@@ -31,6 +31,8 @@ namespace {
 
 constexpr unsigned kBlock = 256;
 constexpr unsigned SF_TJ = 128;   // interactions per plain-FP32 partial sum
+constexpr int kPrefetch = 4;      // bodies per scalar-load group (one s_load_dwordx16)
+static_assert(SF_TJ % kPrefetch == 0, "tiles hold whole prefetch groups");
 
 // One step of Kahan summation (reference src/gravity.c:771-784):
 // y = x - c; t = s + y; c = (t - s) - y; s = t.
@@ -119,31 +121,49 @@ __global__ __launch_bounds__(kBlock) void k_pp_fast(const float4* __restrict__ b
 		Cx[k] = Cy[k] = Cz[k] = Cp[k] = 0.0f;
 	}
 
+	// The j stream keeps one group of kPrefetch bodies in flight: the scalar
+	// loads of the next group are issued before the current group's
+	// arithmetic, so their latency overlaps it.
 	const unsigned j0 = s * slice_len;
 	const unsigned j1 = min(j0 + slice_len, n_pad);
+	float4 nb[kPrefetch];
+#pragma unroll
+	for (int u = 0; u < kPrefetch; ++u) nb[u] = body[j0 + u];
 	for (unsigned jt = j0; jt < j1; jt += SF_TJ) {
 		float tx[IPT], ty[IPT], tz[IPT], tp[IPT];
 #pragma unroll
 		for (int k = 0; k < IPT; ++k) tx[k] = ty[k] = tz[k] = tp[k] = 0.0f;
-#pragma unroll 4
-		for (unsigned j = jt; j < jt + SF_TJ; ++j) {
-			const float4 bj = body[j];
+		for (unsigned j = jt; j < jt + SF_TJ; j += kPrefetch) {
+			float4 cb[kPrefetch];
 #pragma unroll
-			for (int k = 0; k < IPT; ++k) {
-				const float dx = bj.x - xi[k];
-				const float dy = bj.y - yi[k];
-				const float dz = bj.z - zi[k];
-				float r2 = __builtin_fmaf(dx, dx, eps2);
-				r2 = __builtin_fmaf(dy, dy, r2);
-				r2 = __builtin_fmaf(dz, dz, r2);
-				const float inv = __builtin_amdgcn_rsqf(r2);
-				const float mi = bj.w * inv;
-				const float inv2 = inv * inv;
-				const float w = mi * inv2;
-				tx[k] = __builtin_fmaf(w, dx, tx[k]);
-				ty[k] = __builtin_fmaf(w, dy, ty[k]);
-				tz[k] = __builtin_fmaf(w, dz, tz[k]);
-				if (WITH_POT) tp[k] += mi;
+			for (int u = 0; u < kPrefetch; ++u) cb[u] = nb[u];
+			// the group after the slice's last is its own first again (in bounds;
+			// the compiler then issues the four loads as separate dwordx4 into
+			// fresh registers, early in the group -- measured faster than one
+			// dwordx16 into the registers being consumed)
+			const unsigned jn = (j + kPrefetch < j1) ? j + kPrefetch : j;
+#pragma unroll
+			for (int u = 0; u < kPrefetch; ++u) nb[u] = body[jn + u];
+#pragma unroll
+			for (int u = 0; u < kPrefetch; ++u) {
+				const float4 bj = cb[u];
+#pragma unroll
+				for (int k = 0; k < IPT; ++k) {
+					const float dx = bj.x - xi[k];
+					const float dy = bj.y - yi[k];
+					const float dz = bj.z - zi[k];
+					float r2 = __builtin_fmaf(dx, dx, eps2);
+					r2 = __builtin_fmaf(dy, dy, r2);
+					r2 = __builtin_fmaf(dz, dz, r2);
+					const float inv = __builtin_amdgcn_rsqf(r2);
+					const float mi = bj.w * inv;
+					const float inv2 = inv * inv;
+					const float w = mi * inv2;
+					tx[k] = __builtin_fmaf(w, dx, tx[k]);
+					ty[k] = __builtin_fmaf(w, dy, ty[k]);
+					tz[k] = __builtin_fmaf(w, dz, tz[k]);
+					if (WITH_POT) tp[k] += mi;
+				}
 			}
 		}
 #pragma unroll
