@@ -164,6 +164,58 @@ def bench_bh(steps: int = 3, pp_compare: bool = True) -> dict:
     return out
 
 
+def _kdk_run(N, steps, *, ic, theta=None, leaf=8, eps=EPS, stride=None):
+    """steps KDK steps from the IC in the fast policy; FP64 E = T + W before
+    and after, and the relative force error of the final state against the
+    FP64 direct sum on every stride-th particle."""
+    kw = dict(logging=0, epsilon=eps, dt=DT, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL,
+              mem_mode=L.SF_MEM_RESIDENT)
+    if theta is None:
+        s = Simulation(N, enable_tree=0, **kw)
+    else:
+        s = Simulation(N, enable_tree=1, tree_mode=L.SF_TREE_LBVH, theta=theta, leaf_size=leaf, **kw)
+    ic(s)
+    s.solve()
+    e0 = s.stats()
+    hip_sync()
+    t0 = time.perf_counter()
+    for n in range(steps):
+        s.step_number = n
+        s.step()
+    hip_sync()
+    step_ms = (time.perf_counter() - t0) / steps * 1e3
+    e1 = s.stats()
+    s.to_host()
+    idx = np.arange(0, N, stride or max(1, N // 4096), dtype=np.uint32)
+    ref, _ = s.direct_f64(idx)
+    err = rel_err_pcts(s.acc[idx].astype(np.float64), ref)
+    s.close()
+    E0, E1 = e0["K"] + e0["W"], e1["K"] + e1["W"]
+    return {"step_ms": step_ms, "force_err_vs_fp64": err, "energy_rel_drift": abs(E1 - E0) / abs(E0),
+            "E0": E0, "E1": E1, "error_sample": f"every {stride or max(1, N // 4096)}th particle"}
+
+
+def bench_bh1m() -> dict:
+    """Barnes-Hut on a Plummer sphere N = 1048576 (a = 1, M = 1, seeded),
+    theta = 0.5, eps = 0.005, dt = 1e-3, 20 KDK steps; force error against the
+    FP64 direct sum on every 256th particle."""
+    r = _kdk_run(1048576, 20, ic=lambda s: s.ic_plummer(seed=3, first=0, count=s.N), theta=0.5, eps=5e-3, stride=256)
+    r["workload"] = "BH theta=0.5, Plummer N=1048576, eps=0.005, dt=1e-3, 20 KDK steps, leaf 8"
+    return r
+
+
+def bench_stress() -> dict:
+    """Precision and scale: PP at N = 262144 and BH with theta in {0.3, 0.5,
+    0.7} at N = 4194304 (Plummer a = 1, seeded, eps = 0.002), 10 KDK steps
+    each: error percentiles against the FP64 direct sum and energy drift."""
+    out = {"workload": "Plummer a=1, eps=0.002, dt=1e-3, 10 KDK steps, fast policy; errors vs FP64 direct sum"}
+    out["pp_262144"] = _kdk_run(262144, 10, ic=lambda s: s.ic_plummer(seed=5, first=0, count=s.N), stride=64)
+    for th in (0.3, 0.5, 0.7):
+        out[f"bh_4194304_theta{th}"] = _kdk_run(4194304, 10, ic=lambda s: s.ic_plummer(seed=5, first=0, count=s.N),
+                                                theta=th, stride=1024)
+    return out
+
+
 def bench_copy(mib: int = 1024) -> dict:
     """A plain device copy of known size (torch), the calibration run for the
     FETCH_SIZE / WRITE_SIZE counters."""
@@ -251,8 +303,8 @@ def bench_sph(steps: int = 50) -> dict:
     lib.sf_prof_enable(0)
     e1 = s.stats()
     s.close()
-    E0 = e0["T"] + e0["W"] + e0["U"]
-    E1 = e1["T"] + e1["W"] + e1["U"]
+    E0 = e0["K"] + e0["W"] + e0["U"]
+    E1 = e1["K"] + e1["W"] + e1["U"]
     return {"workload": "PP gravity + PP cubic-spline SPH, uniform ball N=65536, gamma=5/3, u=0.05, h=0.05, "
                         "eps=0.01, dt=5e-4, 50 KDK steps, fast policy, state resident",
             "step_ms": step_ms, "sph_density_ms": rho_ms / max(rho_n, 1), "sph_force_ms": frc_ms / max(frc_n, 1),
@@ -267,7 +319,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the BH, adjoint and SPH measurements")
-    ap.add_argument("--only", choices=["bh", "adjoint", "sph", "copy"],
+    ap.add_argument("--only", choices=["bh", "adjoint", "sph", "copy", "bh1m", "stress"],
                     help="run one secondary workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -276,7 +328,8 @@ def main():
         sys.exit(2)
     if args.only:
         fn = {"bh": lambda: bench_bh(steps=args.steps, pp_compare=False), "adjoint": lambda: bench_adjoint(reps=1),
-              "sph": lambda: bench_sph(steps=args.steps), "copy": bench_copy}[args.only]
+              "sph": lambda: bench_sph(steps=args.steps), "copy": bench_copy, "bh1m": bench_bh1m,
+              "stress": bench_stress}[args.only]
         print(json.dumps(fn()))
         return
 
@@ -325,7 +378,7 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": PP_FAST_TRAFFIC_BYTES,
                      "kernel": "k_pp_fast", "kernel_ms": pp_avg_s * 1e3,
                      "flops_per_launch": FLOPS_PER_INTERACTION * inter},
-        "energy_check": {"E": st["E"], "T": st["T"], "W": st["W"]},
+        "energy_check": {"K": st["K"], "W": st["W"], "T_ken": st["T"]},
     }
     if not args.no_extra:
         out["extra"] = {"bh_4m": bench_bh(), "adjoint_16k": bench_adjoint(), "sph_65k": bench_sph()}

@@ -145,6 +145,8 @@ typedef struct {
 	double U;        /* sum m_i u_i (SF_SPH_CUBIC_IDEAL), else 0 */
 	double P[3];     /* linear momentum sum m v (FP64) */
 	double L[3];     /* angular momentum sum m x cross v (FP64) */
+	double K;        /* 0.5 sum m |v|^2 of the current velocities (FP64); T is the
+	                    sum of the per-particle energies a step last wrote */
 } sf_stats_t;
 /* Statistics of the current device state (the T, V of the last sim_step
    are the values it logged). */

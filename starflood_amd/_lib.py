@@ -61,7 +61,7 @@ class Params(C.Structure):
 class Stats(C.Structure):
     """sf_stats_t."""
     _fields_ = [("T", C.c_double), ("V", C.c_double), ("E", C.c_double), ("W", C.c_double), ("U", C.c_double),
-                ("P", C.c_double * 3), ("L", C.c_double * 3)]
+                ("P", C.c_double * 3), ("L", C.c_double * 3), ("K", C.c_double)]
 
 
 # (name, restype, argtypes) for every exported entry point

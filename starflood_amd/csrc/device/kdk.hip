@@ -88,7 +88,7 @@ __global__ __launch_bounds__(kBlock) void k_kick_fast(float* __restrict__ vel, c
 	if (k < n3) vel[k] = __builtin_fmaf(half_dt, acc[k], vel[k]);
 }
 
-constexpr int kRedQ = 10;   // T, V, W, U, Px, Py, Pz, Lx, Ly, Lz
+constexpr int kRedQ = 11;   // T, V, W, U, Px, Py, Pz, Lx, Ly, Lz, K
 
 // Per-block partial sums with a fixed grid-stride assignment and a fixed
 // LDS tree, so the result does not depend on scheduling.
@@ -121,6 +121,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_state(const float* __restrict
 		q[7] += m * (y * vz - z * vy);
 		q[8] += m * (z * vx - x * vz);
 		q[9] += m * (x * vy - y * vx);
+		q[10] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
 	}
 #pragma unroll
 	for (int t = 0; t < kRedQ; ++t) sh[t][threadIdx.x] = q[t];

@@ -240,9 +240,9 @@ int sim_step(sim_t* SF_RESTRICT sim_ptr) {
 
 	double* red_out = c->d_red + (size_t)(sfd_reduce_scratch_doubles(N) - SF_RED_VALUES);
 	if (exact) {
-		if (sfd_energy_sum_sequential(ken, N, red_out + 10, s)) return STARFLOOD_FAILURE;   /* ken_sum */
+		if (sfd_energy_sum_sequential(ken, N, red_out + 12, s)) return STARFLOOD_FAILURE;   /* ken_sum */
 		if (timing) tev_record(5, s);
-		if (sfd_energy_sum_sequential(pen, N, red_out + 11, s)) return STARFLOOD_FAILURE;   /* pen_sum */
+		if (sfd_energy_sum_sequential(pen, N, red_out + 13, s)) return STARFLOOD_FAILURE;   /* pen_sum */
 		if (timing) tev_record(6, s);
 	}
 	/* FP64 W, P, L (and T, V in the fast policy) */
@@ -265,8 +265,8 @@ int sim_step(sim_t* SF_RESTRICT sim_ptr) {
 		if (sf_finish(c, "sim_step")) return STARFLOOD_FAILURE;
 		sf_stats_t* st = &c->stats;
 		const double* r = c->h_red;
-		st->T = exact ? r[10] : r[0];
-		st->V = exact ? r[11] : r[1];
+		st->T = exact ? r[12] : r[0];
+		st->V = exact ? r[13] : r[1];
 		st->E = st->V + st->T;
 		st->W = r[2];
 		st->U = r[3];
@@ -274,6 +274,7 @@ int sim_step(sim_t* SF_RESTRICT sim_ptr) {
 			st->P[k] = r[4 + k];
 			st->L[k] = r[7 + k];
 		}
+		st->K = r[10];
 		if (p->logging && g_log_statistics) {
 			fprintf(g_log_statistics, "%u,%.015f,%.015f,%.015f\n", sim.step_number, st->V + st->T, st->V, st->T);
 			fflush(g_log_statistics);
@@ -315,15 +316,15 @@ int sf_get_stats(const sim_t* sim_ptr, sf_stats_t* out) {
 	                     c->have_u ? c->d_u : NULL, N, p->G, p->epsilon, c->d_red, s))
 		return STARFLOOD_FAILURE;
 	if (exact) {
-		if (sfd_energy_sum_sequential(ken, N, red_out + 10, s)) return STARFLOOD_FAILURE;
-		if (sfd_energy_sum_sequential(pen, N, red_out + 11, s)) return STARFLOOD_FAILURE;
+		if (sfd_energy_sum_sequential(ken, N, red_out + 12, s)) return STARFLOOD_FAILURE;
+		if (sfd_energy_sum_sequential(pen, N, red_out + 13, s)) return STARFLOOD_FAILURE;
 	}
 	SF_HIP(hipMemcpyAsync(c->h_red, red_out, SF_RED_VALUES * sizeof(double), hipMemcpyDeviceToHost, s),
 	       "sf_get_stats");
 	if (sf_finish(c, "sf_get_stats")) return STARFLOOD_FAILURE;
 	const double* r = c->h_red;
-	out->T = exact ? r[10] : r[0];
-	out->V = exact ? r[11] : r[1];
+	out->T = exact ? r[12] : r[0];
+	out->V = exact ? r[13] : r[1];
 	out->E = out->T + out->V;
 	out->W = r[2];
 	out->U = r[3];
@@ -331,5 +332,6 @@ int sf_get_stats(const sim_t* sim_ptr, sf_stats_t* out) {
 		out->P[k] = r[4 + k];
 		out->L[k] = r[7 + k];
 	}
+	out->K = r[10];
 	return STARFLOOD_SUCCESS;
 }

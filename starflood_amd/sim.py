@@ -144,7 +144,7 @@ class Simulation:
     def stats(self) -> dict:
         s = L.Stats()
         L.check(self.lib.sf_get_stats(C.byref(self.sim), C.byref(s)), "sf_get_stats")
-        return {"T": s.T, "V": s.V, "E": s.E, "W": s.W, "U": s.U, "P": list(s.P), "L": list(s.L)}
+        return {"T": s.T, "V": s.V, "E": s.E, "W": s.W, "U": s.U, "P": list(s.P), "L": list(s.L), "K": s.K}
 
     def ic_uniform_ball(self, seed: int, radius: float = 1.0, total_mass: float = 1.0, v_hubble: float = 0.0):
         L.check(self.lib.sf_ic_uniform_ball(C.byref(self.sim), seed, radius, total_mass, v_hubble),

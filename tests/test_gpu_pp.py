@@ -182,3 +182,22 @@ def test_logging_writes_reference_csv_schema(tmp_path):
     tl = (tmp_path / "timings_sim_step.csv").read_text().splitlines()
     assert tl[0] == "step_number,kick_0,drift,sim_solv,kick_1,ken_sum,pen_sum" and len(tl) == 4
     L.set_params()
+
+
+def test_stats_kinetic_energy_and_momenta_match_host_fp64():
+    """sf_get_stats: K = 0.5 sum m v^2 of the current velocities (also before
+    any step has written ken), momentum and angular momentum, in FP64."""
+    N = 3000
+    with exact_sim(N, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL) as s:
+        s.ic_plummer(seed=4, first=0, count=N)
+        s.solve()
+        for k in range(2):
+            st = s.stats()
+            m = s.mas.astype(np.float64)
+            v = s.vel.astype(np.float64)
+            x = s.pos.astype(np.float64)
+            K = 0.5 * (m * (v * v).sum(1)).sum()
+            assert st["K"] == pytest.approx(K, rel=1e-12)
+            assert np.allclose(st["P"], (m[:, None] * v).sum(0), rtol=1e-9, atol=1e-15)
+            assert np.allclose(st["L"], (m[:, None] * np.cross(x, v)).sum(0), rtol=1e-7, atol=1e-12)
+            s.step()

@@ -146,7 +146,7 @@ def test_cubic_model_conserves_momentum_and_energy_rate():
 def test_cubic_gas_sphere_conserves_energy_and_momentum():
     """A cold-ish gas ball with self-gravity (the PP + SPH step of the
     N=65536 workload at N=8192): 40 KDK steps at dt = 5e-4 keep
-    E = T + W + U within 1e-4 relative and the momentum at the FP64 rounding
+    E = K + W + U within 1e-4 relative and the momentum at the FP64 rounding
     floor of its sum."""
     N = 8192
     pos, vel, mas, rad, u = cubic_state(N, 11, 0.1)
@@ -161,12 +161,12 @@ def test_cubic_gas_sphere_conserves_energy_and_momentum():
         s.set_u(u)
         s.solve()
         e0 = s.stats()
-        E0 = e0["T"] + e0["W"] + e0["U"]
+        E0 = e0["K"] + e0["W"] + e0["U"]
         for n in range(40):
             s.step_number = n
             s.step()
         e1 = s.stats()
-        E1 = e1["T"] + e1["W"] + e1["U"]
+        E1 = e1["K"] + e1["W"] + e1["U"]
         assert abs(E1 - E0) <= 1e-4 * abs(E0)
         assert e1["U"] != e0["U"]                      # the gas did work
         assert np.abs(e1["P"]).max() <= 1e-6
