@@ -112,7 +112,7 @@ def bench_bh(steps: int = 3, pp_compare: bool = True) -> dict:
     """Barnes-Hut (SF_TREE_LBVH, theta = 0.5) KDK step at N = 4194304 against
     one fast PP force evaluation of the same state; force errors of both
     against the FP64 direct sum on every 1024th particle."""
-    N, theta, leaf = 4194304, 0.5, 8
+    N, theta, leaf = 4194304, 0.5, 4
     common = dict(logging=0, epsilon=EPS, dt=DT, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL,
                   mem_mode=L.SF_MEM_RESIDENT)
     s = Simulation(N, enable_tree=1, tree_mode=L.SF_TREE_LBVH, theta=theta, leaf_size=leaf, **common)
@@ -164,7 +164,7 @@ def bench_bh(steps: int = 3, pp_compare: bool = True) -> dict:
     return out
 
 
-def _kdk_run(N, steps, *, ic, theta=None, leaf=8, eps=EPS, stride=None):
+def _kdk_run(N, steps, *, ic, theta=None, leaf=4, eps=EPS, stride=None):
     """steps KDK steps from the IC in the fast policy; FP64 E = T + W before
     and after, and the relative force error of the final state against the
     FP64 direct sum on every stride-th particle."""

@@ -122,3 +122,84 @@ def test_lbvh_kdk_steps_and_reference_acc_mode(oracle):
             s.step()
         assert np.isfinite(s.pos).all() and np.isfinite(s.acc).all()
     L.set_params()
+
+
+def degenerate_cases(N, rng):
+    """(name, positions) pairs that stress the Morton build: coincident
+    particles (equal keys, split by index), a line (zero extent in y and z),
+    two clusters 1e4 apart, and a grid (many equal-prefix ties)."""
+    dup = rng.normal(size=(N, 3))
+    dup[: N // 2] = dup[0]                             # half the particles at one point
+    line = np.zeros((N, 3))
+    line[:, 0] = rng.uniform(-1, 1, N)
+    far = rng.normal(scale=0.1, size=(N, 3))
+    far[N // 2:, 0] += 1.0e4
+    g = int(round(N ** (1 / 3)))
+    ax = np.arange(g) / g
+    grid = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+    grid = np.concatenate([grid, rng.uniform(size=(N - len(grid), 3))])[:N]
+    return [("duplicates", dup), ("line", line), ("two_clusters", far), ("grid", grid)]
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_degenerate_distributions_equal_direct_sum_when_every_node_opens(oracle, case):
+    N = 3000
+    rng = np.random.default_rng(77 + case)
+    name, pos = degenerate_cases(N, rng)[case]
+    with lbvh_sim(N, theta=1e-6) as s:
+        s.pos[:] = pos.astype(np.float32)
+        s.mas[:] = rng.uniform(0.5, 1.5, N).astype(np.float32) / N
+        s.solve()
+        ex = s.lbvh_export()
+        assert ex["bad"] == 0, name
+        if name != "duplicates":   # coincident particles form cells of the finest size,
+            # which distant groups accept: their monopole is exact
+            assert ex["cells"] == 0 and ex["parts"] == N * N, name
+        keys = oracle.morton63(s.pos.copy())
+        assert np.array_equal(ex["keys"], np.sort(keys)), name
+        assert np.array_equal(ex["order"], np.argsort(keys, kind="stable").astype(np.uint32)), name
+        a64, _ = oracle.pp_f64(s.pos.copy(), s.mas.copy(), 1e-2, acc_phys=1)
+        scale = np.linalg.norm(a64, axis=1).max() + 1e-30
+        assert np.abs(s.acc - a64).max() <= 2e-6 * scale, name
+    L.set_params()
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_degenerate_distributions_at_theta_half(oracle, case):
+    """The accepted-cell path on the same distributions: finite forces, no
+    malformed-tree report, and the monopole truncation error of a normal
+    walk.  The error is measured against the RMS force, since on a line
+    the net force nearly cancels for many particles (measured: median
+    |da| / rms|a| 6.5e-3 on the line, below 1.5e-3 elsewhere)."""
+    N = 20000
+    rng = np.random.default_rng(91 + case)
+    name, pos = degenerate_cases(N, rng)[case]
+    with lbvh_sim(N, theta=0.5, leaf_size=8) as s:
+        s.pos[:] = pos.astype(np.float32)
+        s.mas[:] = np.full(N, 1.0 / N, np.float32)
+        s.solve()
+        assert s.lbvh_export()["bad"] == 0, name
+        assert np.isfinite(s.acc).all(), name
+        idx = np.arange(0, N, 7)
+        a64, _ = oracle.pp_f64(s.pos.copy(), s.mas.copy(), 1e-2, acc_phys=1, idx=idx.astype(np.uint32))
+        d = np.linalg.norm(s.acc[idx] - a64, axis=1)
+        rms = np.sqrt((np.linalg.norm(a64, axis=1) ** 2).mean())
+        assert np.median(d) / rms < 1e-2, (name, np.median(d) / rms)
+        if name != "line":
+            assert np.median(rel_err(s.acc[idx], a64)) < 5e-3, name
+    L.set_params()
+
+
+def test_mirror_and_resident_modes_give_identical_forces():
+    N = 30000
+    outs = []
+    for mode in (L.SF_MEM_MIRROR, L.SF_MEM_RESIDENT):
+        with lbvh_sim(N, mem_mode=mode) as s:
+            s.ic_plummer(seed=13, first=0, count=N)
+            s.solve()
+            s.step()
+            if mode == L.SF_MEM_RESIDENT:
+                s.to_host()
+            outs.append(s.mem.copy())
+    assert np.array_equal(bits(outs[0]), bits(outs[1]))
+    L.set_params()
