@@ -108,44 +108,56 @@ def rel_err_pcts(acc, ref):
             "p99": float(np.percentile(e, 99)), "max": float(e.max())}
 
 
+BH_RUNS = {
+    # monopole cells: the reference tree's expansion
+    "monopole_theta0.5": dict(theta=0.5, leaf_size=4, quadrupole=0),
+    # quadrupole cells at the theta whose 99th-percentile force error is below 1e-3
+    "quadrupole_theta0.55": dict(theta=0.55, leaf_size=8, quadrupole=1),
+}
+
+
 def bench_bh(steps: int = 3, pp_compare: bool = True) -> dict:
-    """Barnes-Hut (SF_TREE_LBVH, theta = 0.5) KDK step at N = 4194304 against
-    one fast PP force evaluation of the same state; force errors of both
-    against the FP64 direct sum on every 1024th particle."""
-    N, theta, leaf = 4194304, 0.5, 4
+    """Barnes-Hut (SF_TREE_LBVH) KDK steps at N = 4194304 (Plummer a = 1,
+    seeded), monopole at theta = 0.5 and quadrupole at theta = 0.55, against
+    one fast PP force evaluation of the same state; force errors of all
+    three against the FP64 direct sum on every 1024th particle."""
+    N = 4194304
     common = dict(logging=0, epsilon=EPS, dt=DT, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL,
                   mem_mode=L.SF_MEM_RESIDENT)
-    s = Simulation(N, enable_tree=1, tree_mode=L.SF_TREE_LBVH, theta=theta, leaf_size=leaf, **common)
-    lib = s.lib
-    s.ic_plummer(seed=21, first=0, count=N)
-    s.solve()
-    s.step()
-    hip_sync()
-    lib.sf_prof_enable(1)
-    lib.sf_prof_reset()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        s.step()
-    hip_sync()
-    step_ms = (time.perf_counter() - t0) / steps * 1e3
-    b_ms, b_n = kernel_ms(lib, "lbvh_build")
-    w_ms, w_n = kernel_ms(lib, "lbvh_walk")
-    lib.sf_prof_enable(0)
-    ex = s.lbvh_export()
-    s.to_host()
-    pos = s.pos.copy()
-    mas = s.mas.copy()
+    out = {"workload": f"Plummer sphere N={N} (a=1, M=1, seed 21), eps={EPS}, dt={DT}, KDK step in the fast policy, "
+                       "state resident", "N": N, "error_sample": "every 1024th particle (4096)"}
     idx = np.arange(0, N, 1024, dtype=np.uint32)
-    ref, _ = s.direct_f64(idx)
-    bh_err = rel_err_pcts(s.acc[idx].astype(np.float64), ref)
-    s.close()
-    walk_ms = w_ms / max(w_n, 1)
-    inter = float(ex["cells"] + ex["parts"])
-    out = {"workload": f"Plummer sphere N={N} (a=1, M=1, seed 21), theta={theta}, leaf_size={leaf}, "
-                       f"eps={EPS}, dt={DT}, KDK step in the fast policy, state resident",
-           "N": N, "theta": theta, "step_ms": step_ms, "build_ms": b_ms / max(b_n, 1), "walk_ms": walk_ms,
-           "interactions_per_particle": inter / N, "walk_interactions_per_s": inter / (walk_ms / 1e3),
-           "force_err_vs_fp64": bh_err, "error_sample": "every 1024th particle (4096)"}
+    pos = mas = ref = None
+    for name, cfg in BH_RUNS.items():
+        s = Simulation(N, enable_tree=1, tree_mode=L.SF_TREE_LBVH, **cfg, **common)
+        lib = s.lib
+        s.ic_plummer(seed=21, first=0, count=N)
+        s.solve()
+        s.step()
+        hip_sync()
+        lib.sf_prof_enable(1)
+        lib.sf_prof_reset()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.step()
+        hip_sync()
+        step_ms = (time.perf_counter() - t0) / steps * 1e3
+        b_ms, b_n = kernel_ms(lib, "lbvh_build")
+        w_ms, w_n = kernel_ms(lib, "lbvh_walk")
+        lib.sf_prof_enable(0)
+        ex = s.lbvh_export()
+        s.to_host()
+        run_ref, _ = s.direct_f64(idx)   # each run's own final state
+        if ref is None:
+            pos, mas, ref = s.pos.copy(), s.mas.copy(), run_ref
+        walk_ms = w_ms / max(w_n, 1)
+        inter = float(ex["cells"] + ex["parts"])
+        out[name] = {**cfg, "step_ms": step_ms, "build_ms": b_ms / max(b_n, 1), "walk_ms": walk_ms,
+                     "interactions_per_particle": inter / N, "walk_interactions_per_s": inter / (walk_ms / 1e3),
+                     "force_err_vs_fp64": rel_err_pcts(s.acc[idx].astype(np.float64), run_ref)}
+        s.close()
+        L.set_params()
+    # PP is timed and checked on the first run's final state
     if not pp_compare:
         return out
     p = Simulation(N, enable_tree=0, **common)
@@ -160,7 +172,10 @@ def bench_bh(steps: int = 3, pp_compare: bool = True) -> dict:
     p.to_host()
     pp_err = rel_err_pcts(p.acc[idx].astype(np.float64), ref)
     p.close()
-    out.update({"pp_force_ms": pp_ms, "pp_force_err_vs_fp64": pp_err, "pp_force_over_bh_step": pp_ms / step_ms})
+    out["pp_force_ms"] = pp_ms
+    out["pp_force_err_vs_fp64"] = pp_err
+    for name in BH_RUNS:
+        out[name]["pp_force_over_bh_step"] = pp_ms / out[name]["step_ms"]
     return out
 
 

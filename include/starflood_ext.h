@@ -100,6 +100,9 @@ typedef struct {
 	                                (config.h:120-123) (1) */
 	int verbose;                 /* print energies per step (simulation.c:503) (0) */
 	char out_dir[256];           /* OUTPUT_DIR, config.h:75 ("./out") */
+	int quadrupole;              /* SF_TREE_LBVH cells carry traceless quadrupoles
+	                                as well as monopoles (0: monopoles, as the
+	                                reference's tree) */
 } sf_params_t;
 
 /* Fill *p with the defaults listed above. */

@@ -304,75 +304,177 @@ __global__ __launch_bounds__(kBlock) void k_karras(const unsigned long long* __r
 	node_range[i] = make_int2(lo, hi - lo + 1);
 }
 
-// ---- FP64 inclusive prefix sums of (m, m x, m y, m z) in sorted order.
+// ---- FP64 inclusive prefix sums over the sorted particles of (m, m x, m y,
+// m z) and, for quadrupoles, of m x' x'^T (x' = x - ref, six components).
 // Fixed shape: 16 consecutive elements per thread, a Hillis-Steele scan of
 // the 256 thread sums per block, one block scanning the block sums.
 struct d4 {
 	double m, x, y, z;
+	__device__ static d4 zero() { return {0.0, 0.0, 0.0, 0.0}; }
+	__device__ static d4 of(float4 b, const double*) {
+		const double m = b.w;
+		return {m, m * (double)b.x, m * (double)b.y, m * (double)b.z};
+	}
+	__device__ d4 operator+(const d4& o) const { return {m + o.m, x + o.x, y + o.y, z + o.z}; }
 };
-__device__ __forceinline__ d4 d4_add(d4 a, d4 b) { return {a.m + b.m, a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ d4 moment(float4 b) {
-	const double m = b.w;
-	return {m, m * (double)b.x, m * (double)b.y, m * (double)b.z};
-}
+struct d6 {   // xx, xy, xz, yy, yz, zz
+	double v[6];
+	__device__ static d6 zero() { return {{0.0, 0.0, 0.0, 0.0, 0.0, 0.0}}; }
+	__device__ static d6 of(float4 b, const double* ref) {
+		const double m = b.w, x = (double)b.x - ref[0], y = (double)b.y - ref[1], z = (double)b.z - ref[2];
+		return {{m * x * x, m * x * y, m * x * z, m * y * y, m * y * z, m * z * z}};
+	}
+	__device__ d6 operator+(const d6& o) const {
+		d6 r;
+		for (int k = 0; k < 6; ++k) r.v[k] = v[k] + o.v[k];
+		return r;
+	}
+};
 
 constexpr unsigned kScanPer = 16;
 constexpr unsigned kScanTile = kBlock * kScanPer;
 
-__device__ d4 block_exclusive_scan(d4 v, d4* sh) {
+template <class R>
+__device__ R block_exclusive_scan(R v, R* sh) {
 	const unsigned t = threadIdx.x;
 	sh[t] = v;
 	__syncthreads();
 	for (unsigned off = 1; off < kBlock; off <<= 1) {
-		const d4 o = t >= off ? sh[t - off] : d4{0.0, 0.0, 0.0, 0.0};
+		const R o = t >= off ? sh[t - off] : R::zero();
 		__syncthreads();
-		sh[t] = d4_add(sh[t], o);
+		sh[t] = sh[t] + o;
 		__syncthreads();
 	}
-	const d4 excl = t ? sh[t - 1] : d4{0.0, 0.0, 0.0, 0.0};
+	const R excl = t ? sh[t - 1] : R::zero();
 	__syncthreads();
 	return excl;
 }
 
+template <class R>
 __global__ __launch_bounds__(kBlock) void k_scan_tiles(const float4* __restrict__ body, unsigned N,
-                                                       d4* __restrict__ tile_sum) {
-	__shared__ d4 sh[kBlock];
+                                                       const double* __restrict__ ref, R* __restrict__ tile_sum) {
+	__shared__ R sh[kBlock];
 	const unsigned b0 = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
-	d4 s = {0.0, 0.0, 0.0, 0.0};
+	R s = R::zero();
 	for (unsigned k = 0; k < kScanPer; ++k)
-		if (b0 + k < N) s = d4_add(s, moment(body[b0 + k]));
-	const d4 excl = block_exclusive_scan(s, sh);
-	if (threadIdx.x == kBlock - 1) tile_sum[blockIdx.x] = d4_add(excl, s);
+		if (b0 + k < N) s = s + R::of(body[b0 + k], ref);
+	const R excl = block_exclusive_scan(s, sh);
+	if (threadIdx.x == kBlock - 1) tile_sum[blockIdx.x] = excl + s;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_tile_sums(d4* __restrict__ tile_sum, unsigned ntiles) {
-	__shared__ d4 sh[kBlock];
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_scan_tile_sums(R* __restrict__ tile_sum, unsigned ntiles) {
+	__shared__ R sh[kBlock];
 	const unsigned chunk = (ntiles + kBlock - 1) / kBlock;
 	const unsigned b = threadIdx.x * chunk, e = min(b + chunk, ntiles);
-	d4 s = {0.0, 0.0, 0.0, 0.0};
-	for (unsigned k = b; k < e; ++k) s = d4_add(s, tile_sum[k]);
-	d4 run = block_exclusive_scan(s, sh);
+	R s = R::zero();
+	for (unsigned k = b; k < e; ++k) s = s + tile_sum[k];
+	R run = block_exclusive_scan(s, sh);
 	for (unsigned k = b; k < e; ++k) {
-		const d4 v = tile_sum[k];
+		const R v = tile_sum[k];
 		tile_sum[k] = run;       // exclusive offset of tile k
-		run = d4_add(run, v);
+		run = run + v;
 	}
 }
 
+template <class R>
 __global__ __launch_bounds__(kBlock) void k_scan_write(const float4* __restrict__ body, unsigned N,
-                                                       const d4* __restrict__ tile_off, d4* __restrict__ prefix) {
-	__shared__ d4 sh[kBlock];
+                                                       const double* __restrict__ ref, const R* __restrict__ tile_off,
+                                                       R* __restrict__ prefix) {
+	__shared__ R sh[kBlock];
 	const unsigned b0 = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
-	d4 s = {0.0, 0.0, 0.0, 0.0};
+	R s = R::zero();
 	for (unsigned k = 0; k < kScanPer; ++k)
-		if (b0 + k < N) s = d4_add(s, moment(body[b0 + k]));
-	d4 run = d4_add(tile_off[blockIdx.x], block_exclusive_scan(s, sh));
+		if (b0 + k < N) s = s + R::of(body[b0 + k], ref);
+	R run = tile_off[blockIdx.x] + block_exclusive_scan(s, sh);
 	for (unsigned k = 0; k < kScanPer; ++k) {
 		if (b0 + k < N) {
-			run = d4_add(run, moment(body[b0 + k]));
+			run = run + R::of(body[b0 + k], ref);
 			prefix[b0 + k] = run;   // inclusive
 		}
 	}
+}
+
+// Reference point of the second-moment prefix sums: the global centre of
+// mass (the last first-moment prefix), so that the sums stay of the order of
+// the system's own second moment (a point far from the mass, such as the
+// centre of a bounding cube stretched by one distant particle, would make
+// them large and the differences S - M c c^T of small nodes meaningless).
+__global__ void k_quad_ref(const d4* __restrict__ prefix, unsigned N, double* __restrict__ ref) {
+	if (threadIdx.x >= 3) return;
+	const d4 t = prefix[N - 1];
+	const double c = threadIdx.x == 0 ? t.x : (threadIdx.x == 1 ? t.y : t.z);
+	ref[threadIdx.x] = t.m > 0.0 ? c / t.m : 0.0;
+}
+
+// Traceless quadrupole Qt = 3 Q - tr(Q) I of every internal node about its
+// centre of mass, Q = sum m (x - c)(x - c)^T; stored as q0 = (Qxx, Qxy, Qxz,
+// Qyy), q1 = Qyz (Qzz = -Qxx - Qyy).  Nodes of at most kQuadDirect particles
+// sum directly in FP64 about their own centre of mass (each particle lies in
+// a handful of such nodes); larger nodes use the prefix differences
+// S - M c c^T, whose rounding is small next to their Q.  Single-particle
+// leaves have Qt = 0.
+constexpr int kQuadDirect = 256;
+
+__global__ __launch_bounds__(kBlock) void k_node_quad(int N, const int2* __restrict__ node_range,
+                                                      const float4* __restrict__ body, const d4* __restrict__ prefix,
+                                                      const d6* __restrict__ prefix2, const double* __restrict__ ref,
+                                                      float4* __restrict__ q0, float* __restrict__ q1) {
+	const int n = (int)(blockIdx.x * kBlock + threadIdx.x);
+	if (n >= 2 * N - 1) return;
+	if (n >= N - 1) {
+		q0[n] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		q1[n] = 0.0f;
+		return;
+	}
+	const int2 rg = node_range[n];
+	const int f = rg.x, l = rg.x + rg.y - 1;
+	const d4 a = f > 0 ? prefix[f - 1] : d4::zero();
+	const d6 a2 = f > 0 ? prefix2[f - 1] : d6::zero();
+	const d4 b = prefix[l];
+	const d6 b2 = prefix2[l];
+	const double M = b.m - a.m;
+	if (!(M > 0.0)) {
+		q0[n] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		q1[n] = 0.0f;
+		return;
+	}
+	double qxx, qxy, qxz, qyy, qyz, qzz;
+	if (rg.y <= kQuadDirect) {
+		double sm = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+		for (int k = f; k <= l; ++k) {
+			const float4 p = body[k];
+			sm += (double)p.w;
+			sx += (double)p.w * (double)p.x;
+			sy += (double)p.w * (double)p.y;
+			sz += (double)p.w * (double)p.z;
+		}
+		const double cx = sx / sm, cy = sy / sm, cz = sz / sm;
+		qxx = qxy = qxz = qyy = qyz = qzz = 0.0;
+		for (int k = f; k <= l; ++k) {
+			const float4 p = body[k];
+			const double m = p.w, x = (double)p.x - cx, y = (double)p.y - cy, z = (double)p.z - cz;
+			qxx += m * x * x;
+			qxy += m * x * y;
+			qxz += m * x * z;
+			qyy += m * y * y;
+			qyz += m * y * z;
+			qzz += m * z * z;
+		}
+	} else {
+		// centre of mass relative to ref
+		const double cx = (b.x - a.x) / M - ref[0], cy = (b.y - a.y) / M - ref[1], cz = (b.z - a.z) / M - ref[2];
+		qxx = (b2.v[0] - a2.v[0]) - M * cx * cx;
+		qxy = (b2.v[1] - a2.v[1]) - M * cx * cy;
+		qxz = (b2.v[2] - a2.v[2]) - M * cx * cz;
+		qyy = (b2.v[3] - a2.v[3]) - M * cy * cy;
+		qyz = (b2.v[4] - a2.v[4]) - M * cy * cz;
+		qzz = (b2.v[5] - a2.v[5]) - M * cz * cz;
+	}
+	const double tr = qxx + qyy + qzz;
+	q0[n] = make_float4((float)(3.0 * qxx - tr), (float)(3.0 * qxy), (float)(3.0 * qxz), (float)(3.0 * qyy - tr));
+	q1[n] = (float)(3.0 * qyz);
+	(void)qzz;
 }
 
 __device__ __forceinline__ unsigned compact21(unsigned long long v) {
@@ -402,7 +504,7 @@ __global__ __launch_bounds__(kBlock) void k_node_moments(const float4* __restric
 	const int2 rg = node_range[n];
 	const int f = rg.x, l = rg.x + rg.y - 1;
 	const d4 hi = prefix[l];
-	const d4 lo = f > 0 ? prefix[f - 1] : d4{0.0, 0.0, 0.0, 0.0};
+	const d4 lo = f > 0 ? prefix[f - 1] : d4::zero();
 	const double M = hi.m - lo.m;
 	// cell of the common code prefix: p leading bits of the 63-bit code
 	const unsigned long long ka = key[f], kb = key[l];
@@ -558,14 +660,36 @@ int sfd_lbvh_topology(const float* pos, const float* mas, const unsigned long lo
 	if (N > 1)
 		hipLaunchKernelGGL(k_karras, dim3((N - 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, key, (int)N,
 		                   static_cast<int2*>(node_lr), rg);
-	hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s, b4, N, ts);
-	hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kBlock), 0, s, ts, ntiles);
-	hipLaunchKernelGGL(k_scan_write, dim3(ntiles), dim3(kBlock), 0, s, b4, N, ts, pre);
+	hipLaunchKernelGGL(k_scan_tiles<d4>, dim3(ntiles), dim3(kBlock), 0, s, b4, N, (const double*)nullptr, ts);
+	hipLaunchKernelGGL(k_scan_tile_sums<d4>, dim3(1), dim3(kBlock), 0, s, ts, ntiles);
+	hipLaunchKernelGGL(k_scan_write<d4>, dim3(ntiles), dim3(kBlock), 0, s, b4, N, (const double*)nullptr, ts, pre);
 	const unsigned nn = 2u * N - 1u;
 	hipLaunchKernelGGL(k_node_moments, dim3((nn + kBlock - 1) / kBlock), dim3(kBlock), 0, s, b4, (int)N, key, rg, pre,
 	                   box, static_cast<float4*>(node_cm), static_cast<float4*>(node_cell));
 	sf_prof_end(tok, s);
 	return sf_hip_check(hipGetLastError(), "sfd_lbvh_topology", "launch");
+}
+
+// Traceless quadrupoles of every node (after sfd_lbvh_topology): prefix2 is
+// N records of 6 doubles, tile_sum2 sfd_lbvh_scan_tiles(N) of them, ref 3
+// doubles; node_q0 / node_q1 are 2N-1 float4 / float.
+int sfd_lbvh_quadrupoles(unsigned N, const void* body, const void* node_range, const void* prefix, const double* box,
+                         void* prefix2, void* tile_sum2, double* ref, void* node_q0, float* node_q1, hipStream_t s) {
+	(void)box;
+	if (N == 0) return 0;
+	const float4* b4 = static_cast<const float4*>(body);
+	d6* pre2 = static_cast<d6*>(prefix2);
+	d6* ts2 = static_cast<d6*>(tile_sum2);
+	const unsigned ntiles = sfd_lbvh_scan_tiles(N);
+	hipLaunchKernelGGL(k_quad_ref, dim3(1), dim3(64), 0, s, static_cast<const d4*>(prefix), N, ref);
+	hipLaunchKernelGGL(k_scan_tiles<d6>, dim3(ntiles), dim3(kBlock), 0, s, b4, N, (const double*)ref, ts2);
+	hipLaunchKernelGGL(k_scan_tile_sums<d6>, dim3(1), dim3(kBlock), 0, s, ts2, ntiles);
+	hipLaunchKernelGGL(k_scan_write<d6>, dim3(ntiles), dim3(kBlock), 0, s, b4, N, (const double*)ref, ts2, pre2);
+	const unsigned nn = 2u * N - 1u;
+	hipLaunchKernelGGL(k_node_quad, dim3((nn + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (int)N,
+	                   static_cast<const int2*>(node_range), b4, static_cast<const d4*>(prefix), pre2,
+	                   (const double*)ref, static_cast<float4*>(node_q0), node_q1);
+	return sf_hip_check(hipGetLastError(), "sfd_lbvh_quadrupoles", "launch");
 }
 
 int sfd_lbvh_records(unsigned N, const void* node_lr, const void* node_range, const void* node_cm,

@@ -90,11 +90,54 @@ __device__ __forceinline__ void eval_list(Acc& a, float xi, float yi, float zi, 
 	fold(a);
 }
 
+// Monopole plus traceless quadrupole of a cell with e = com - x:
+//   acc += M e / r^3 - Qt e / r^5 + (5/2) (e^T Qt e) e / r^7
+//   pot += M / r + (1/2) (e^T Qt e) / r^5
+// (the potential -G [M/r + (1/2) d^T Qt d / r^5] with d = -e and its
+// gradient), Plummer-softened through r^2 = |e|^2 + eps^2.  Qt is
+// (q0 = Qxx, Qxy, Qxz, Qyy; q1 = Qyz), Qzz = -Qxx - Qyy.
+__device__ __forceinline__ void interact_quad(Acc& a, float xi, float yi, float zi, float4 c, float4 q0, float q1,
+                                              float eps2) {
+	const float ex = c.x - xi, ey = c.y - yi, ez = c.z - zi;
+	float r2 = __builtin_fmaf(ex, ex, eps2);
+	r2 = __builtin_fmaf(ey, ey, r2);
+	r2 = __builtin_fmaf(ez, ez, r2);
+	const float inv = __builtin_amdgcn_rsqf(r2);
+	const float inv2 = inv * inv;
+	const float mi = c.w * inv;
+	const float inv3 = inv * inv2;
+	const float inv5 = inv3 * inv2;
+	const float qzz = -q0.x - q0.w;
+	const float qx = __builtin_fmaf(q0.x, ex, __builtin_fmaf(q0.y, ey, q0.z * ez));
+	const float qy = __builtin_fmaf(q0.y, ex, __builtin_fmaf(q0.w, ey, q1 * ez));
+	const float qz = __builtin_fmaf(q0.z, ex, __builtin_fmaf(q1, ey, qzz * ez));
+	const float eqe = __builtin_fmaf(ex, qx, __builtin_fmaf(ey, qy, ez * qz));
+	const float w = __builtin_fmaf(2.5f * eqe * inv5, inv2, c.w * inv3);
+	a.tx = __builtin_fmaf(w, ex, __builtin_fmaf(-inv5, qx, a.tx));
+	a.ty = __builtin_fmaf(w, ey, __builtin_fmaf(-inv5, qy, a.ty));
+	a.tz = __builtin_fmaf(w, ez, __builtin_fmaf(-inv5, qz, a.tz));
+	a.tp += __builtin_fmaf(0.5f * inv5, eqe, mi);
+}
+
+__device__ __forceinline__ void eval_list_quad(Acc& a, float xi, float yi, float zi, const float4* c, const float4* q0,
+                                               const float* q1, int n, float eps2) {
+	if (n == 64) {
+#pragma unroll 4
+		for (int k = 0; k < 64; ++k) interact_quad(a, xi, yi, zi, c[k], q0[k], q1[k], eps2);
+	} else {
+		for (int k = 0; k < n; ++k) interact_quad(a, xi, yi, zi, c[k], q0[k], q1[k], eps2);
+	}
+	fold(a);
+}
+
 // 5 wavefronts per SIMD: the register budget that keeps the walk spill-free
-// (88 VGPRs) while raising occupancy from 4 (measured 7% faster at 4M)
+// (88 VGPRs) while raising occupancy from 4 (measured 7% faster at 4M).
+// QUAD: cells carry traceless quadrupoles (node_q0, node_q1).
+template <bool QUAD>
 __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ body, const unsigned* __restrict__ order,
                                                  int N, const unsigned* __restrict__ group_start,
                                                  const float4* __restrict__ rec_a,
+                                                 const float4* __restrict__ node_q0, const float* __restrict__ node_q1,
                                                  const int4* __restrict__ rec_b, float eps2, double G, int acc_phys,
                                                  float* __restrict__ acc, float* __restrict__ pot,
                                                  unsigned long long* __restrict__ counters, int* __restrict__ gstack,
@@ -103,7 +146,11 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
 	__shared__ int2 s_leaf[kWaves][kList];
 	__shared__ int s_pid[kWaves][kList];
 	__shared__ float4 s_part[kWaves][64];
+	__shared__ float4 s_q0[kWaves][QUAD ? kList : 1];
+	__shared__ float s_q1[kWaves][QUAD ? kList : 1];
 	const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+	float4* cq0 = s_q0[w];
+	float* cq1 = s_q1[w];
 	int* stack = gstack + (size_t)(blockIdx.x * kWaves + w) * kStack;
 	float4* cells = s_cell[w];
 	int2* leaves = s_leaf[w];
@@ -181,7 +228,14 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
 					const unsigned long long m_cell = __ballot(cls == 1);
 					const unsigned long long m_leaf = __ballot(cls == 2);
 					const unsigned long long m_open = __ballot(cls == 3);
-					if (cls == 1) cells[nc + __popcll(m_cell & lt)] = ra;
+					if (cls == 1) {
+						const int at = nc + __popcll(m_cell & lt);
+						cells[at] = ra;
+						if (QUAD) {
+							cq0[at] = node_q0[node];
+							cq1[at] = node_q1[node];
+						}
+					}
 					if (cls == 2) leaves[nl + __popcll(m_leaf & lt)] = make_int2(rb.y, rb.z);
 					if (cls == 3) {
 						const int at = sp + 2 * __popcll(m_open & lt);
@@ -205,10 +259,17 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
 				// monopoles
 				if (nc >= 64 || (drain && nc > 0)) {
 					const int take = min(nc, 64);
-					eval_list(a, xi, yi, zi, cells, take, eps2);
+					if (QUAD) eval_list_quad(a, xi, yi, zi, cells, cq0, cq1, take, eps2);
+					else eval_list(a, xi, yi, zi, cells, take, eps2);
 					n_cell += (unsigned long long)take;
 					wave_fence();
-					if ((int)lane < nc - take) cells[lane] = cells[take + lane];
+					if ((int)lane < nc - take) {
+						cells[lane] = cells[take + lane];
+						if (QUAD) {
+							cq0[lane] = cq0[take + lane];
+							cq1[lane] = cq1[take + lane];
+						}
+					}
 					nc -= take;
 					wave_fence();
 				}
@@ -272,12 +333,13 @@ __global__ void k_walk_reset(unsigned long long* __restrict__ counters) {
 
 // Blocks that are resident at once: every wavefront of the grid runs from the
 // start, so the static group split is balanced (cached after the first call).
-unsigned resident_blocks(void) {
+template <bool QUAD>
+unsigned resident_blocks_of(void) {
 	static unsigned cached = 0;
 	if (cached) return cached;
 	int per_cu = 0, cus = 0, dev = 0;
 	if (hipGetDevice(&dev) != hipSuccess ||
-	    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_walk, kBlock, 0) != hipSuccess ||
+	    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_walk<QUAD>, kBlock, 0) != hipSuccess ||
 	    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || per_cu < 1 ||
 	    cus < 1) {
 		(void)hipGetLastError();
@@ -288,6 +350,9 @@ unsigned resident_blocks(void) {
 	return cached;
 }
 
+// the node stacks are sized for the larger of the two variants' grids
+unsigned resident_blocks(void) { return max(resident_blocks_of<false>(), resident_blocks_of<true>()); }
+
 }  // namespace
 
 extern "C" {
@@ -295,18 +360,28 @@ extern "C" {
 size_t sfd_lbvh_walk_stack_bytes(void) { return (size_t)resident_blocks() * kWaves * kStack * sizeof(int); }
 
 int sfd_lbvh_walk(const void* body, const unsigned* order, unsigned N, const unsigned* group_start,
-                  const void* node_cm, const void* rec_b, float eps2, double G, int acc_phys, float* acc, float* pot,
-                  unsigned long long* counters, int* gstack, hipStream_t s) {
+                  const void* node_cm, const void* node_q0, const float* node_q1, const void* rec_b, float eps2,
+                  double G, int acc_phys, float* acc, float* pot, unsigned long long* counters, int* gstack,
+                  hipStream_t s) {
 	if (N == 0) return 0;
 	if (!counters || !gstack || !group_start) return -1;
+	const bool quad = node_q0 && node_q1;
 	const unsigned chunks = (N + 63) / 64;   // group_start holds 64 * (chunks + 1) flags
 	unsigned blocks = (chunks + kWaves - 1) / kWaves;
-	if (blocks > resident_blocks()) blocks = resident_blocks();   // gstack holds resident_blocks() slots
+	const unsigned resident = quad ? resident_blocks_of<true>() : resident_blocks_of<false>();
+	if (blocks > resident) blocks = resident;   // gstack holds resident_blocks() >= resident slots
 	hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, s, counters);
 	const int tok = sf_prof_begin("lbvh_walk", s);
-	hipLaunchKernelGGL(k_walk, dim3(blocks), dim3(kBlock), 0, s, static_cast<const float4*>(body), order, (int)N,
-	                   group_start, static_cast<const float4*>(node_cm), static_cast<const int4*>(rec_b), eps2, G,
-	                   acc_phys, acc, pot, counters, gstack, chunks);
+	if (quad)
+		hipLaunchKernelGGL(k_walk<true>, dim3(blocks), dim3(kBlock), 0, s, static_cast<const float4*>(body), order,
+		                   (int)N, group_start, static_cast<const float4*>(node_cm),
+		                   static_cast<const float4*>(node_q0), node_q1, static_cast<const int4*>(rec_b), eps2, G,
+		                   acc_phys, acc, pot, counters, gstack, chunks);
+	else
+		hipLaunchKernelGGL(k_walk<false>, dim3(blocks), dim3(kBlock), 0, s, static_cast<const float4*>(body), order,
+		                   (int)N, group_start, static_cast<const float4*>(node_cm), (const float4*)nullptr,
+		                   (const float*)nullptr, static_cast<const int4*>(rec_b), eps2, G, acc_phys, acc, pot,
+		                   counters, gstack, chunks);
 	sf_prof_end(tok, s);
 	return sf_hip_check(hipGetLastError(), "sfd_lbvh_walk", "launch");
 }

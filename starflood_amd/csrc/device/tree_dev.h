@@ -58,8 +58,16 @@ int sfd_lbvh_records(unsigned int N, const void* node_lr, const void* node_range
    gstack: sfd_lbvh_walk_stack_bytes() of scratch. */
 size_t sfd_lbvh_walk_stack_bytes(void);
 int sfd_lbvh_walk(const void* body, const unsigned int* order, unsigned int N, const unsigned int* group_start,
-                  const void* node_cm, const void* rec_b, float eps2, double G, int acc_phys, float* acc, float* pot,
-                  unsigned long long* counters, int* gstack, hipStream_t s);
+                  const void* node_cm, const void* node_q0, const float* node_q1, const void* rec_b, float eps2,
+                  double G, int acc_phys, float* acc, float* pot, unsigned long long* counters, int* gstack,
+                  hipStream_t s);
+/* traceless quadrupoles (node_q0 2N-1 float4, node_q1 2N-1 float) from a
+   second FP64 prefix scan (prefix2: N x 6 doubles, tile_sum2:
+   sfd_lbvh_scan_tiles(N) x 6 doubles, ref: 3 doubles); NULL node_q0 in
+   sfd_lbvh_walk selects the monopole walk */
+int sfd_lbvh_quadrupoles(unsigned int N, const void* body, const void* node_range, const void* prefix,
+                         const double* box, void* prefix2, void* tile_sum2, double* ref, void* node_q0,
+                         float* node_q1, hipStream_t s);
 
 /* walk groups: start[p] = 1 where a maximal subtree of at most cap (<= 64)
    particles begins in sorted order, and start[N] = 1; start holds

@@ -35,12 +35,19 @@ struct sf_lbvh {
 	unsigned long long* h_counters; /* pinned */
 	int* gstack;                    /* walk node stacks, one per wavefront slot */
 	unsigned int* group_start;      /* walk group starts in sorted order */
+	/* quadrupoles (allocated on first use) */
+	void* prefix2;                  /* N x 6 doubles: inclusive m x' x'^T sums */
+	void* tile_sum2;
+	double* quad_ref;               /* 3 doubles: reference point of the sums */
+	void* node_q0;                  /* 2N-1 float4 */
+	float* node_q1;                 /* 2N-1 floats */
+	int quad_valid;                 /* the last build computed them */
 };
 
 static void free_buffers(sf_lbvh* t) {
 	void* ptrs[] = {t->key0, t->key1, t->val0, t->val1, t->hist, t->bbox_part, t->box, t->body, t->node_lr,
 	                t->node_range, t->prefix, t->tile_sum, t->node_cm, t->node_cell, t->rec_b, t->counters,
-	                t->gstack, t->group_start};
+	                t->gstack, t->group_start, t->prefix2, t->tile_sum2, t->quad_ref, t->node_q0, t->node_q1};
 	for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k)
 		if (ptrs[k]) hipFree(ptrs[k]);
 	if (t->h_counters) hipHostFree(t->h_counters);
@@ -98,13 +105,28 @@ static int ensure(sf_lbvh** st, unsigned int N) {
 	return 0;
 }
 
-int sf_lbvh_build(sf_lbvh** st, unsigned int N, const float* pos, const float* mas, hipStream_t s) {
+static int ensure_quad(sf_lbvh* t) {
+	if (t->node_q0) return 0;
+	const size_t N = t->n_alloc, nn = 2u * N;
+	int rc = 0;
+	rc |= alloc(&t->prefix2, N * 48u);
+	rc |= alloc(&t->tile_sum2, (size_t)sfd_lbvh_scan_tiles((unsigned int)N) * 48u);
+	rc |= alloc((void**)&t->quad_ref, 3u * 8u);
+	rc |= alloc(&t->node_q0, nn * 16u);
+	rc |= alloc((void**)&t->node_q1, nn * 4u);
+	if (rc) sf_error("tree_build", "device allocation for LBVH quadrupoles of %zu particles failed", N);
+	return rc ? -1 : 0;
+}
+
+int sf_lbvh_build(sf_lbvh** st, unsigned int N, const float* pos, const float* mas, int quadrupole, hipStream_t s) {
 	if (N == 0) {
 		if (*st) (*st)->N = 0;
 		return 0;
 	}
 	if (ensure(st, N)) return -1;
 	sf_lbvh* t = *st;
+	t->quad_valid = 0;
+	if (quadrupole && ensure_quad(t)) return -1;
 	const unsigned int tile = sfd_radix_tile();
 	t->n_sort = (N + tile - 1u) / tile * tile;
 	t->N = N;
@@ -115,6 +137,12 @@ int sf_lbvh_build(sf_lbvh** st, unsigned int N, const float* pos, const float* m
 	                      t->tile_sum, t->node_cm, t->node_cell, s))
 		return -1;
 	if (sfd_lbvh_groups(N, t->node_lr, t->node_range, SF_LBVH_GROUP, t->group_start, s)) return -1;
+	if (quadrupole) {
+		if (sfd_lbvh_quadrupoles(N, t->body, t->node_range, t->prefix, t->box, t->prefix2, t->tile_sum2, t->quad_ref,
+		                         t->node_q0, t->node_q1, s))
+			return -1;
+		t->quad_valid = 1;
+	}
 	sf_prof_end(tok, s);
 	return 0;
 }
@@ -131,8 +159,14 @@ int sf_lbvh_walk(sf_lbvh* t, unsigned int N, const float* pos, const float* mas,
 	if (sfd_lbvh_records(N, t->node_lr, t->node_range, t->node_cm, t->node_cell, (float)p->theta, (int)p->leaf_size,
 	                     t->rec_b, s))
 		return -1;
-	return sfd_lbvh_walk(t->body, t->val0, N, t->group_start, t->node_cm, t->rec_b, (float)(p->epsilon * p->epsilon),
-	                     p->G, p->acc_mode == SF_ACC_PHYSICAL, acc, pot, t->counters, t->gstack, s);
+	const int quad = p->quadrupole && t->quad_valid;
+	if (p->quadrupole && !t->quad_valid) {
+		sf_error("solve_gravity_part_tree", "quadrupole is set but the tree was built without quadrupoles");
+		return -1;
+	}
+	return sfd_lbvh_walk(t->body, t->val0, N, t->group_start, t->node_cm, quad ? t->node_q0 : NULL,
+	                     quad ? t->node_q1 : NULL, t->rec_b, (float)(p->epsilon * p->epsilon), p->G,
+	                     p->acc_mode == SF_ACC_PHYSICAL, acc, pot, t->counters, t->gstack, s);
 }
 
 int sf_lbvh_export_impl(sf_lbvh* t, hipStream_t s, unsigned long long* keys, unsigned int* order,

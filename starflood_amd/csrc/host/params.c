@@ -37,6 +37,7 @@ void sf_params_default(sf_params_t* p) {
 	p->logging = 1;                 /* config.h:120-123 */
 	p->verbose = 0;
 	strcpy(p->out_dir, "./out");    /* config.h:75 */
+	p->quadrupole = 0;
 }
 
 const sf_params_t* sf_params(void) {
@@ -73,6 +74,10 @@ int sf_set_params(const sf_params_t* p) {
 	    (p->sph_model != SF_SPH_REFERENCE && p->sph_model != SF_SPH_CUBIC_IDEAL) ||
 	    (p->mem_mode != SF_MEM_MIRROR && p->mem_mode != SF_MEM_RESIDENT)) {
 		sf_error("sf_set_params", "unknown enumeration value");
+		return STARFLOOD_FAILURE;
+	}
+	if (p->quadrupole != 0 && p->quadrupole != 1) {
+		sf_error("sf_set_params", "quadrupole must be 0 or 1");
 		return STARFLOOD_FAILURE;
 	}
 	if (memchr(p->out_dir, 0, sizeof(p->out_dir)) == NULL) {

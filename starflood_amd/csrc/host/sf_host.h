@@ -33,7 +33,7 @@ int sf_solve_all(sf_ctx* c, tree_t* tree, unsigned int step_number, float sph_pr
 
 /* lbvh.c: SF_TREE_LBVH build (allocates *st on first use) and walk */
 typedef struct sf_lbvh sf_lbvh;
-int sf_lbvh_build(sf_lbvh** st, unsigned int N, const float* pos, const float* mas, hipStream_t s);
+int sf_lbvh_build(sf_lbvh** st, unsigned int N, const float* pos, const float* mas, int quadrupole, hipStream_t s);
 int sf_lbvh_walk(sf_lbvh* st, unsigned int N, const float* pos, const float* mas, const sf_params_t* p, float* acc,
                  float* pot, hipStream_t s);
 void sf_lbvh_destroy(sf_lbvh* st);

@@ -185,7 +185,7 @@ int tree_build(tree_t* SF_RESTRICT tree_ptr, unsigned int num_parts, const real*
 	if (p->tree_mode == SF_TREE_REFERENCE) {
 		rc = build_ref(d, tree_ptr, num_parts, dpos, dmas, s);
 	} else {
-		rc = sf_lbvh_build(&d->lbvh, num_parts, dpos, dmas, s);
+		rc = sf_lbvh_build(&d->lbvh, num_parts, dpos, dmas, sf_params()->quadrupole, s);
 		if (!rc) tree_ptr->num_nodes = num_parts ? 2u * num_parts - 1u : 0u;
 		if (hipStreamSynchronize(s) != hipSuccess) rc = -1;
 	}
@@ -225,7 +225,7 @@ int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number) {
 			return -1;
 		return rc;
 	}
-	if (sf_lbvh_build(&d->lbvh, N, pos, mas, s)) return -1;
+	if (sf_lbvh_build(&d->lbvh, N, pos, mas, p->quadrupole, s)) return -1;
 	tree->num_nodes = N ? 2u * N - 1u : 0u;
 	return sf_lbvh_walk(d->lbvh, N, pos, mas, p, acc, pot, s);
 }

@@ -152,8 +152,10 @@ def test_degenerate_distributions_equal_direct_sum_when_every_node_opens(oracle,
         s.solve()
         ex = s.lbvh_export()
         assert ex["bad"] == 0, name
-        if name != "duplicates":   # coincident particles form cells of the finest size,
-            # which distant groups accept: their monopole is exact
+        if name in ("line", "grid"):
+            # elsewhere nodes of equal Morton keys have the finest cell size (coincident
+            # particles; 0.005-wide cells 1e4 from the other cluster), which distant groups
+            # rightly accept even at theta = 1e-6: the force check below still holds
             assert ex["cells"] == 0 and ex["parts"] == N * N, name
         keys = oracle.morton63(s.pos.copy())
         assert np.array_equal(ex["keys"], np.sort(keys)), name
@@ -202,4 +204,90 @@ def test_mirror_and_resident_modes_give_identical_forces():
                 s.to_host()
             outs.append(s.mem.copy())
     assert np.array_equal(bits(outs[0]), bits(outs[1]))
+    L.set_params()
+
+
+def quad_cell_field(src_pos, src_m, x, eps):
+    """FP64 monopole + traceless-quadrupole field and potential of the
+    point masses src at points x (the expansion about their centre of mass,
+    Plummer-softened like the device: r^2 = |e|^2 + eps^2)."""
+    M = src_m.sum()
+    c = (src_m[:, None] * src_pos).sum(0) / M
+    d = src_pos - c
+    Q = (src_m[:, None, None] * d[:, :, None] * d[:, None, :]).sum(0)
+    Qt = 3.0 * Q - np.trace(Q) * np.eye(3)
+    e = c[None, :] - x
+    r2 = (e * e).sum(1) + eps * eps
+    inv = 1.0 / np.sqrt(r2)
+    qe = e @ Qt
+    eqe = (e * qe).sum(1)
+    acc = (M * inv ** 3 + 2.5 * eqe * inv ** 7)[:, None] * e - (inv ** 5)[:, None] * qe
+    pot = M * inv + 0.5 * eqe * inv ** 5
+    return acc, -pot
+
+
+def test_quadrupole_cell_matches_fp64_expansion(oracle):
+    """A compact 64-particle cluster seen from a far 64-particle group is
+    accepted as one cell: the device field on the group equals the FP64
+    monopole + quadrupole expansion of the cluster plus the group's own
+    direct sum."""
+    rng = np.random.default_rng(5)
+    a = rng.uniform(0.0, 0.1, size=(64, 3))
+    b = np.array([10.0, 3.0, 2.0]) + rng.uniform(0.0, 0.01, size=(64, 3))
+    pos = np.concatenate([a, b]).astype(np.float32)
+    mas = rng.uniform(0.5, 1.5, 128).astype(np.float32) / 128
+    eps = 1e-2
+    with lbvh_sim(128, theta=0.5, quadrupole=1, epsilon=eps) as s:
+        s.pos[:] = pos
+        s.mas[:] = mas
+        s.solve()
+        assert s.lbvh_export()["cells"] >= 64                # the cluster was accepted for the group
+        p64, m64 = pos.astype(np.float64), mas.astype(np.float64)
+        qa, qp = quad_cell_field(p64[:64], m64[:64], p64[64:], eps)
+        da, dp = oracle.pp_f64(pos[64:].copy(), mas[64:].copy(), eps, acc_phys=1)
+        ref = qa + da
+        got = s.acc[64:].astype(np.float64)
+        assert np.abs(got - ref).max() <= 1e-5 * np.linalg.norm(ref, axis=1).max()
+        assert np.allclose(s.pot[64:], qp + dp, rtol=1e-5)
+    L.set_params()
+
+
+def test_quadrupole_error_at_two_thetas(oracle):
+    """At theta 0.6 and 0.3 the quadrupole walk's median error is at least 4x
+    below the monopole walk's (measured 7.5x on this Plummer sphere), and both
+    fall at least 3x when theta halves (measured ~10x each: with this
+    cell-size criterion the error falls faster than the bare expansion order
+    suggests, for either expansion).  The formula itself is pinned by
+    test_quadrupole_cell_matches_fp64_expansion."""
+    N = 30000
+    med = {}
+    for quad in (0, 1):
+        for theta in (0.6, 0.3):
+            with lbvh_sim(N, theta=theta, quadrupole=quad, leaf_size=4) as s:
+                s.ic_plummer(seed=17, first=0, count=N)
+                s.solve()
+                idx = np.arange(0, N, 10).astype(np.uint32)
+                a64, _ = oracle.pp_f64(s.pos.copy(), s.mas.copy(), 1e-2, acc_phys=1, idx=idx)
+                med[quad, theta] = float(np.median(rel_err(s.acc[idx], a64)))
+    for theta in (0.6, 0.3):
+        assert med[1, theta] < med[0, theta] / 4, med
+    for quad in (0, 1):
+        assert med[quad, 0.6] / med[quad, 0.3] > 3.0, med
+    L.set_params()
+
+
+def test_quadrupole_walk_is_more_accurate_at_the_same_theta(oracle):
+    N = 60000
+    errs = {}
+    for quad in (0, 1):
+        with lbvh_sim(N, theta=0.5, quadrupole=quad, leaf_size=4) as s:
+            s.ic_plummer(seed=23, first=0, count=N)
+            s.solve()
+            assert s.lbvh_export()["bad"] == 0
+            idx = np.arange(0, N, 15).astype(np.uint32)
+            a64, _ = oracle.pp_f64(s.pos.copy(), s.mas.copy(), 1e-2, acc_phys=1, idx=idx)
+            e = rel_err(s.acc[idx], a64)
+            errs[quad] = (np.median(e), np.percentile(e, 99))
+    assert errs[1][0] < errs[0][0] / 3, errs
+    assert errs[1][1] < errs[0][1] / 2, errs
     L.set_params()
