@@ -199,13 +199,17 @@ def kdk_adjoint_f64(x0, v0, m, eps, G, dt, acc_phys, nsteps, target):
     return L, gx.reshape(-1, 3), gv.reshape(-1, 3)
 
 
-def tree_ref(pos, mas, step_number, max_nodes=262144, depth_max=16):
-    """Bounds, build and node array of the reference octree."""
+def tree_ref(pos, mas, step_number, max_nodes=262144, depth_max=16, bounds=None):
+    """Bounds, build and node array of the reference octree (bounds from the
+    step's jittered mean radius unless given)."""
     pos = np.ascontiguousarray(pos, np.float32)
     mas = np.ascontiguousarray(mas, np.float32)
     N = len(mas)
-    bounds = np.zeros(6, np.float32)
-    lib().orc_tree_bounds_ref(N, fp(pos), step_number, fp(bounds))
+    if bounds is None:
+        bounds = np.zeros(6, np.float32)
+        lib().orc_tree_bounds_ref(N, fp(pos), step_number, fp(bounds))
+    else:
+        bounds = np.ascontiguousarray(bounds, np.float32).copy()
     nodes = (OrcNode * max_nodes)()
     nn = C.c_uint(0)
     rc = lib().orc_tree_build_ref(N, fp(pos), fp(mas), fp(bounds), max_nodes, depth_max, nodes, C.byref(nn))

@@ -622,27 +622,56 @@ unsigned sfd_radix_hist_entries(unsigned n_sort) {
 
 unsigned sfd_lbvh_scan_tiles(unsigned N) { return (N + kScanTile - 1) / kScanTile; }
 
-// Sorts n_sort (a multiple of the tile) pairs; the result ends in (key0,
-// val0) after the eight passes.
-int sfd_radix_sort(unsigned long long* key0, unsigned* val0, unsigned long long* key1, unsigned* val1,
-                   unsigned n_sort, unsigned* hist, hipStream_t s) {
+// Histograms of up to this many entries (128 tiles, 524288 keys) are scanned
+// by one block in one launch instead of three.
+constexpr unsigned kRadixScanOneBlock = 32768;
+
+// Sorts n_sort (a multiple of the tile) pairs on the low 8 * passes key bits;
+// the result ends in (key0, val0) after an even number of passes and in
+// (key1, val1) after an odd one.
+int sfd_radix_sort_passes(unsigned long long* key0, unsigned* val0, unsigned long long* key1, unsigned* val1,
+                          unsigned n_sort, unsigned* hist, unsigned passes, hipStream_t s) {
+	if (n_sort % kTile != 0 || passes > 8u) return -1;
 	const unsigned ntiles = n_sort / kTile;
 	const unsigned nh = 256u * ntiles;
 	const unsigned nchunks = (nh + kScanU32Chunk - 1u) / kScanU32Chunk;
 	unsigned* chunk_sums = hist + nh;   // sfd_radix_hist_entries() leaves room for them
-	for (unsigned pass = 0; pass < 8; ++pass) {
+	if (ntiles == 0) return 0;
+	for (unsigned pass = 0; pass < passes; ++pass) {
 		const unsigned shift = 8u * pass;
 		unsigned long long* kin = (pass & 1) ? key1 : key0;
 		unsigned* vin = (pass & 1) ? val1 : val0;
 		unsigned long long* kout = (pass & 1) ? key0 : key1;
 		unsigned* vout = (pass & 1) ? val0 : val1;
 		hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, kin, shift, ntiles, hist);
-		hipLaunchKernelGGL(k_scan_u32_sums, dim3(nchunks), dim3(kBlock), 0, s, hist, nh, chunk_sums);
-		hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, chunk_sums, nchunks);
-		hipLaunchKernelGGL(k_scan_u32_down, dim3(nchunks), dim3(kBlock), 0, s, hist, nh, chunk_sums);
+		if (nh <= kRadixScanOneBlock) {
+			hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, hist, nh);
+		} else {
+			hipLaunchKernelGGL(k_scan_u32_sums, dim3(nchunks), dim3(kBlock), 0, s, hist, nh, chunk_sums);
+			hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, chunk_sums, nchunks);
+			hipLaunchKernelGGL(k_scan_u32_down, dim3(nchunks), dim3(kBlock), 0, s, hist, nh, chunk_sums);
+		}
 		hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, shift, ntiles, hist, kout, vout);
 	}
 	return sf_hip_check(hipGetLastError(), "sfd_radix_sort", "launch");
+}
+
+int sfd_radix_sort(unsigned long long* key0, unsigned* val0, unsigned long long* key1, unsigned* val1,
+                   unsigned n_sort, unsigned* hist, hipStream_t s) {
+	return sfd_radix_sort_passes(key0, val0, key1, val1, n_sort, hist, 8u, s);
+}
+
+unsigned sfd_scan_u32_sums_len(unsigned n) { return (n + kScanU32Chunk - 1u) / kScanU32Chunk; }
+
+// In-place exclusive scan of n u32 entries; sums holds
+// sfd_scan_u32_sums_len(n) entries of scratch.
+int sfd_scan_u32_exclusive(unsigned* data, unsigned n, unsigned* sums, hipStream_t s) {
+	if (n == 0) return 0;
+	const unsigned nchunks = sfd_scan_u32_sums_len(n);
+	hipLaunchKernelGGL(k_scan_u32_sums, dim3(nchunks), dim3(kBlock), 0, s, data, n, sums);
+	hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, sums, nchunks);
+	hipLaunchKernelGGL(k_scan_u32_down, dim3(nchunks), dim3(kBlock), 0, s, data, n, sums);
+	return sf_hip_check(hipGetLastError(), "sfd_scan_u32_exclusive", "launch");
 }
 
 // prefix: N records of 4 doubles; tile_sum: sfd_lbvh_scan_tiles(N) records

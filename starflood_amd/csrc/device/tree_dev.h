@@ -22,12 +22,36 @@ typedef struct {
 } sf_node;
 
 /* --- reference octree (tree_ref.hip) --- */
-int sfd_tree_bounds_ref(const float* pos, unsigned int N, const double jitter[3], float* bounds, hipStream_t s);
+/* r1: N doubles of scratch */
+int sfd_tree_bounds_ref(const float* pos, unsigned int N, const double jitter[3], double* r1, float* bounds,
+                        hipStream_t s);
 /* meta[0] = num_nodes, meta[1] = status (0 ok, 1 out of nodes, 2 too deep) */
 int sfd_tree_build_ref(const float* pos, const float* mas, unsigned int N, const float* bounds, unsigned int max_nodes,
                        int depth_max, sf_node* node, unsigned int* meta, hipStream_t s);
-int sfd_tree_walk_ref(const float* pos, const float* mas, unsigned int N, const sf_node* node, unsigned int node_cap,
-                      unsigned int iter_max,
+/* Level-synchronous build of the same tree (tree_ref_par.hip).  scratch:
+   sfd_tree_par_bytes(N, max_nodes) bytes of device memory (0: only the
+   serial build applies, N > 8 max_nodes); h_state: SF_TREE_PAR_STATE u32 of pinned host
+   memory; side: a second stream and ev: two events for the root's moment
+   sums.  order (N u32) receives a walk order: the particles leaf by leaf,
+   those outside the bounds last.  Returns 0 when node and meta hold the
+   serial build's result, 1 when
+   the build would exceed max_nodes or depth_max (the caller then runs
+   sfd_tree_build_ref, which leaves the partial tree the reference leaves),
+   -1 on a HIP error. */
+#define SF_TREE_PAR_STATE 16
+typedef struct {
+	void* scratch;
+	unsigned int* h_state;
+	hipStream_t side;
+	hipEvent_t ev[2];
+} sf_tree_par_ctx;
+size_t sfd_tree_par_bytes(unsigned int N, unsigned int max_nodes);
+int sfd_tree_build_ref_par(const sf_tree_par_ctx* x, const float* pos, const float* mas, unsigned int N,
+                           const float* bounds, unsigned int max_nodes, int depth_max, sf_node* node,
+                           unsigned int* meta, unsigned int* order, hipStream_t s);
+/* order: the particle of each walk lane (N entries), or NULL for index order */
+int sfd_tree_walk_ref(const float* pos, const float* mas, unsigned int N, const unsigned int* order, const sf_node* node,
+                      unsigned int node_cap, unsigned int iter_max,
                       float eps2, float theta, float negG, int acc_phys, int kahan_acc, int kahan_pot, float* acc,
                       float* pot, hipStream_t s);
 
@@ -44,6 +68,15 @@ int sfd_lbvh_keys(const float* pos, unsigned int N, unsigned int n_sort, double*
 /* stable 8-pass LSD radix sort; the result is in (key0, val0) */
 int sfd_radix_sort(unsigned long long* key0, unsigned int* val0, unsigned long long* key1, unsigned int* val1,
                    unsigned int n_sort, unsigned int* hist, hipStream_t s);
+/* the first `passes` 8-bit passes only: the pairs are sorted on the low
+   8 * passes key bits and end in (key0, val0) for an even pass count,
+   (key1, val1) for an odd one */
+int sfd_radix_sort_passes(unsigned long long* key0, unsigned int* val0, unsigned long long* key1, unsigned int* val1,
+                          unsigned int n_sort, unsigned int* hist, unsigned int passes, hipStream_t s);
+/* in-place exclusive scan of n u32 entries with sfd_scan_u32_sums_len(n)
+   entries of scratch */
+unsigned int sfd_scan_u32_sums_len(unsigned int n);
+int sfd_scan_u32_exclusive(unsigned int* data, unsigned int n, unsigned int* sums, hipStream_t s);
 /* tiles of the FP64 moment scan (records of 4 doubles each) */
 unsigned int sfd_lbvh_scan_tiles(unsigned int N);
 /* sorted bodies, radix tree, FP64 prefix moments (N records of 4 doubles),

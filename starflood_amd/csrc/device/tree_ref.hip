@@ -18,27 +18,58 @@ namespace {
 constexpr unsigned kBlock = 256;
 constexpr int kDepthCap = 16;   // TREE_DEPTH_MAX upper limit (tree_path size)
 
-// Mean radius (sequential FP64 Kahan, src/simulation.c:245-277) and the
-// jittered cube bounds (src/simulation.c:279-301).  jitter[i] is
-// 2 * (INV_PCG32_MAX * s[i]) - 1 from pcg4d of the step number.
-__global__ void k_tree_bounds_ref(const float* __restrict__ pos, unsigned N, double j0, double j1, double j2,
-                                  float* __restrict__ bounds) {
-	if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// Radii r_i = sqrt((x^2 + y^2) + z^2) in FP64 (src/simulation.c:256-262),
+// all particles at once.
+__global__ __launch_bounds__(kBlock) void k_radius_f64(const float* __restrict__ pos, unsigned N,
+                                                       double* __restrict__ r1) {
+	const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= N) return;
+	const double x = pos[3u * i], y = pos[3u * i + 1u], z = pos[3u * i + 2u];
+	r1[i] = sqrt((x * x + y * y) + z * z);
+}
+
+__device__ __forceinline__ void kahan_step(double& avg, double& com, double r) {
+	const double yk = r - com;
+	const double t = avg + yk;
+	const double zk = t - avg;
+	com = zk - yk;
+	avg = t;
+}
+
+// Mean radius (the sequential FP64 Kahan sum of src/simulation.c:245-277,
+// one wavefront; the radii arrive through scalar loads, 16 ahead of the
+// chain) and the jittered cube bounds (src/simulation.c:279-301).
+// jitter[i] is 2 * (INV_PCG32_MAX * s[i]) - 1 from pcg4d of the step number.
+constexpr unsigned kKahanAhead = 16;
+
+__global__ __launch_bounds__(64) void k_tree_bounds_ref(const double* __restrict__ r1, unsigned N, double j0, double j1,
+                                                        double j2, float* __restrict__ bounds) {
 	double avg = 0.0, com = 0.0;
-	for (unsigned i = 0; i < N; ++i) {
-		const double x = pos[3u * i], y = pos[3u * i + 1u], z = pos[3u * i + 2u];
-		const double r1 = sqrt((x * x + y * y) + z * z);
-		const double yk = r1 - com;
-		const double t = avg + yk;
-		const double zk = t - avg;
-		com = zk - yk;
-		avg = t;
+	unsigned i = 0;
+	if (N >= kKahanAhead) {
+		double nx[kKahanAhead];
+#pragma unroll
+		for (unsigned u = 0; u < kKahanAhead; ++u) nx[u] = r1[u];
+		for (; i + 2u * kKahanAhead <= N; i += kKahanAhead) {
+			double cur[kKahanAhead];
+#pragma unroll
+			for (unsigned u = 0; u < kKahanAhead; ++u) cur[u] = nx[u];
+#pragma unroll
+			for (unsigned u = 0; u < kKahanAhead; ++u) nx[u] = r1[i + kKahanAhead + u];
+#pragma unroll
+			for (unsigned u = 0; u < kKahanAhead; ++u) kahan_step(avg, com, cur[u]);
+		}
+#pragma unroll
+		for (unsigned u = 0; u < kKahanAhead; ++u) kahan_step(avg, com, nx[u]);
+		i += kKahanAhead;
 	}
+	for (; i < N; ++i) kahan_step(avg, com, r1[i]);
+	if (threadIdx.x != 0) return;
 	avg *= 1.0 / (double)N;
 	const double jit[3] = {j0, j1, j2};
-	for (int i = 0; i < 3; ++i) {
-		bounds[i] = (float)(-2.000 * avg + 0.5 * jit[i] * avg);
-		bounds[3 + i] = (float)(2.000 * avg + 0.5 * jit[i] * avg);
+	for (int a = 0; a < 3; ++a) {
+		bounds[a] = (float)(-2.000 * avg + 0.5 * jit[a] * avg);
+		bounds[3 + a] = (float)(2.000 * avg + 0.5 * jit[a] * avg);
 	}
 }
 
@@ -158,12 +189,14 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
 // Per-particle walk of src/gravity.c:273-660.
 template <bool KAHAN_ACC, bool KAHAN_POT, bool PHYS>
 __global__ __launch_bounds__(kBlock) void k_tree_walk_ref(const float* __restrict__ pos, const float* __restrict__ mas,
-                                                          unsigned N, const sf_node* __restrict__ node,
+                                                          unsigned N, const unsigned* __restrict__ order,
+                                                          const sf_node* __restrict__ node,
                                                           unsigned node_cap, unsigned iter_max, float eps2,
                                                           float theta, float negG,
                                                           float* __restrict__ acc, float* __restrict__ pot) {
-	const unsigned idx = blockIdx.x * kBlock + threadIdx.x;
-	if (idx >= N) return;
+	const unsigned lane_id = blockIdx.x * kBlock + threadIdx.x;
+	if (lane_id >= N) return;
+	const unsigned idx = order ? order[lane_id] : lane_id;
 	float sx = 0.0f, sy = 0.0f, sz = 0.0f, sp = 0.0f;
 	float cx = 0.0f, cy = 0.0f, cz = 0.0f, cp = 0.0f;
 	const float ri[3] = {pos[3u * idx], pos[3u * idx + 1u], pos[3u * idx + 2u]};
@@ -259,8 +292,11 @@ __global__ __launch_bounds__(kBlock) void k_tree_walk_ref(const float* __restric
 
 }  // namespace
 
-extern "C" int sfd_tree_bounds_ref(const float* pos, unsigned N, const double jitter[3], float* bounds, hipStream_t s) {
-	hipLaunchKernelGGL(k_tree_bounds_ref, dim3(1), dim3(64), 0, s, pos, N, jitter[0], jitter[1], jitter[2], bounds);
+extern "C" int sfd_tree_bounds_ref(const float* pos, unsigned N, const double jitter[3], double* r1, float* bounds,
+                                   hipStream_t s) {
+	if (N == 0) return 0;
+	hipLaunchKernelGGL(k_radius_f64, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pos, N, r1);
+	hipLaunchKernelGGL(k_tree_bounds_ref, dim3(1), dim3(64), 0, s, r1, N, jitter[0], jitter[1], jitter[2], bounds);
 	return sf_hip_check(hipGetLastError(), "sfd_tree_bounds_ref", "launch");
 }
 
@@ -276,15 +312,15 @@ extern "C" int sfd_tree_build_ref(const float* pos, const float* mas, unsigned N
 	return sf_hip_check(hipGetLastError(), "sfd_tree_build_ref", "com launch");
 }
 
-extern "C" int sfd_tree_walk_ref(const float* pos, const float* mas, unsigned N, const sf_node* node,
-                                 unsigned node_cap, unsigned iter_max, float eps2, float theta, float negG, int acc_phys, int kahan_acc,
+extern "C" int sfd_tree_walk_ref(const float* pos, const float* mas, unsigned N, const unsigned* order,
+                                 const sf_node* node, unsigned node_cap, unsigned iter_max, float eps2, float theta, float negG, int acc_phys, int kahan_acc,
                                  int kahan_pot, float* acc, float* pot, hipStream_t s) {
 	if (N == 0) return 0;
 	const dim3 grid((N + kBlock - 1) / kBlock), block(kBlock);
 	const int sel = (kahan_acc ? 4 : 0) | (kahan_pot ? 2 : 0) | (acc_phys ? 1 : 0);
 	const int tok = sf_prof_begin("tree_walk_ref", s);
 #define SF_WALK(A, B, C)                                                                                         \
-	hipLaunchKernelGGL((k_tree_walk_ref<A, B, C>), grid, block, 0, s, pos, mas, N, node, node_cap, iter_max, eps2, theta, \
+	hipLaunchKernelGGL((k_tree_walk_ref<A, B, C>), grid, block, 0, s, pos, mas, N, order, node, node_cap, iter_max, eps2, theta, \
 	                   negG, acc, pot)
 	switch (sel) {
 	case 0: SF_WALK(false, false, false); break;

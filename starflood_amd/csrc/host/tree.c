@@ -30,6 +30,12 @@ struct sf_tree_dev {
 	float* d_stage;          /* staging for host pos (3N) and mas (N) */
 	unsigned int stage_n;
 	hipStream_t stream;      /* for tree_build called directly */
+	sf_tree_par_ctx par;     /* level-synchronous reference build: scratch, */
+	size_t par_bytes;        /* pinned state, side stream and events */
+	double* d_r1;            /* radii for the reference bounds */
+	unsigned int r1_n;
+	unsigned int* d_order;   /* walk order of the last parallel build (N entries) */
+	unsigned int order_n;    /* its N, 0 when the last build was serial */
 	sf_lbvh* lbvh;           /* SF_TREE_LBVH state (lbvh.c) */
 };
 
@@ -55,6 +61,14 @@ static void tree_dev_destroy(sf_tree_dev* d) {
 	if (d->d_meta) hipFree(d->d_meta);
 	if (d->h_meta) hipHostFree(d->h_meta);
 	if (d->d_stage) hipFree(d->d_stage);
+	if (d->par.side) hipStreamSynchronize(d->par.side);
+	if (d->par.scratch) hipFree(d->par.scratch);
+	if (d->par.h_state) hipHostFree(d->par.h_state);
+	for (int k = 0; k < 2; ++k)
+		if (d->par.ev[k]) hipEventDestroy(d->par.ev[k]);
+	if (d->par.side) hipStreamDestroy(d->par.side);
+	if (d->d_r1) hipFree(d->d_r1);
+	if (d->d_order) hipFree(d->d_order);
 	if (d->stream) hipStreamDestroy(d->stream);
 	free(d);
 }
@@ -133,7 +147,37 @@ static int is_device_pointer(const void* p) {
    reference's errors (src/tree.c:180-189) after a synchronisation. */
 static int build_ref(sf_tree_dev* d, tree_t* t, unsigned int N, const float* pos, const float* mas, hipStream_t s) {
 	const sf_params_t* p = sf_params();
-	if (sfd_tree_build_ref(pos, mas, N, d->d_bounds, d->max_nodes, (int)p->tree_depth_max, d->d_node, d->d_meta, s))
+	const size_t need = sfd_tree_par_bytes(N, d->max_nodes);   /* 0: serial build only */
+	d->order_n = 0u;
+	if (need && d->par_bytes < need) {
+		if (d->par.scratch) {
+			SF_HIP(hipStreamSynchronize(s), "tree_build");
+			SF_HIP(hipStreamSynchronize(d->par.side), "tree_build");
+			hipFree(d->par.scratch);
+			hipFree(d->d_order);
+		}
+		d->par.scratch = NULL;
+		d->d_order = NULL;
+		d->par_bytes = 0;
+		SF_HIP(hipMalloc(&d->par.scratch, need), "tree_build");
+		SF_HIP(hipMalloc((void**)&d->d_order, (size_t)(N ? N : 1u) * sizeof(unsigned int)), "tree_build");
+		d->par_bytes = need;
+	}
+	if (need && !d->par.h_state) {
+		SF_HIP(hipHostMalloc((void**)&d->par.h_state, SF_TREE_PAR_STATE * sizeof(unsigned int), hipHostMallocDefault), "tree_build");
+		SF_HIP(hipStreamCreateWithFlags(&d->par.side, hipStreamNonBlocking), "tree_build");
+		for (int k = 0; k < 2; ++k)
+			SF_HIP(hipEventCreateWithFlags(&d->par.ev[k], hipEventDisableTiming), "tree_build");
+	}
+	/* the level-synchronous build; a build that runs out of nodes or depth
+	   is redone by the serial insertion, which stops where the reference does */
+	const int rc = need ? sfd_tree_build_ref_par(&d->par, pos, mas, N, d->d_bounds, d->max_nodes,
+	                                             (int)p->tree_depth_max, d->d_node, d->d_meta, d->d_order, s)
+	                    : 1;
+	if (rc == 0) d->order_n = N;
+	if (rc < 0) return -1;
+	if (rc > 0 &&
+	    sfd_tree_build_ref(pos, mas, N, d->d_bounds, d->max_nodes, (int)p->tree_depth_max, d->d_node, d->d_meta, s))
 		return -1;
 	SF_HIP(hipMemcpyAsync(d->h_meta, d->d_meta, 2u * sizeof(unsigned int), hipMemcpyDeviceToHost, s), "tree_build");
 	SF_HIP(hipStreamSynchronize(s), "tree_build");
@@ -215,12 +259,20 @@ int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number) {
 	if (p->tree_mode == SF_TREE_REFERENCE) {
 		double jit[4];
 		bounds_jitter(step_number, jit);
-		if (N && sfd_tree_bounds_ref(pos, N, jit, d->d_bounds, s)) return -1;
+		if (d->r1_n < N) {
+			SF_HIP(hipStreamSynchronize(s), "solve_gravity_part_tree");
+			if (d->d_r1) hipFree(d->d_r1);
+			d->d_r1 = NULL;
+			d->r1_n = 0;
+			SF_HIP(hipMalloc((void**)&d->d_r1, (size_t)N * sizeof(double)), "solve_gravity_part_tree");
+			d->r1_n = N;
+		}
+		if (N && sfd_tree_bounds_ref(pos, N, jit, d->d_r1, d->d_bounds, s)) return -1;
 		const int rc = build_ref(d, tree, N, pos, mas, s);
 		SF_HIP(hipMemcpyAsync(tree->bounds, d->d_bounds, 6u * sizeof(float), hipMemcpyDeviceToHost, s),
 		       "solve_gravity_part_tree");
 		/* the reference walks the tree even after a failed build */
-		if (sfd_tree_walk_ref(pos, mas, N, d->d_node, d->max_nodes, 2u * 8u * p->tree_nodes_max, eps2, (float)p->theta,
+		if (sfd_tree_walk_ref(pos, mas, N, d->order_n == N ? d->d_order : NULL, d->d_node, d->max_nodes, 2u * 8u * p->tree_nodes_max, eps2, (float)p->theta,
 		                      (float)(-p->G), phys, p->kahan_solver, p->kahan_energy, acc, pot, s))
 			return -1;
 		return rc;
@@ -277,7 +329,7 @@ int solve_gravity_part_tree(const sim_t* SF_RESTRICT sim_ptr) {
 	float* pot = sf_blk(c->d_mem, N, SF_E_POT);
 	int rc;
 	if (p->tree_mode == SF_TREE_REFERENCE) {
-		rc = sfd_tree_walk_ref(pos, mas, N, d->d_node, d->max_nodes, 2u * 8u * p->tree_nodes_max, (float)(p->epsilon * p->epsilon),
+		rc = sfd_tree_walk_ref(pos, mas, N, d->order_n == N ? d->d_order : NULL, d->d_node, d->max_nodes, 2u * 8u * p->tree_nodes_max, (float)(p->epsilon * p->epsilon),
 		                       (float)p->theta, (float)(-p->G), p->acc_mode == SF_ACC_PHYSICAL, p->kahan_solver,
 		                       p->kahan_energy, acc, pot, s);
 	} else {

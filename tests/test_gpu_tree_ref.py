@@ -1,5 +1,7 @@
 """The reference octree on the device (SF_TREE_REFERENCE): node array, walk,
 and a tree step bit-identical to the reference's."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -49,3 +51,90 @@ def test_tree_matches_oracle_on_plummer_and_reports_overflow(oracle):
         assert rc == -1 and s.sim.tree.num_nodes == nn
         assert np.array_equal(s.tree_nodes(), raw)
     L.set_params()
+
+
+def direct_build(pos, mas, bounds, max_nodes=262144):
+    """tree_init / tree_build / tree_free through the C ABI, with the caller's
+    bounds (src/tree.c:74-318): return code, num_nodes and the node bytes."""
+    lib = L.load()
+    t = L.TreeT()
+    assert lib.tree_init(C.byref(t), max_nodes) == 0
+    try:
+        for k in range(6):
+            t.bounds[k] = float(bounds[k])
+        pos = np.ascontiguousarray(pos, np.float32)
+        mas = np.ascontiguousarray(mas, np.float32)
+        rc = lib.tree_build(C.byref(t), len(mas), pos.ctypes.data, mas.ctypes.data)
+        n = t.num_nodes
+        raw = np.ctypeslib.as_array(C.cast(t.node, C.POINTER(C.c_uint8)), shape=(n * 80,)).copy()
+        return rc, n, raw
+    finally:
+        lib.tree_free(C.byref(t))
+
+
+def plummer(N, seed):
+    rng = np.random.default_rng(seed)
+    r = 1.0 / np.sqrt(rng.random(N) ** (-2.0 / 3.0) - 1.0)
+    d = rng.normal(size=(N, 3))
+    pos = (d / np.linalg.norm(d, axis=1)[:, None] * r[:, None]).astype(np.float32)
+    mas = rng.uniform(0.5, 1.5, N).astype(np.float32) / N
+    return pos, mas
+
+
+def test_parallel_build_on_split_planes_matches_oracle(oracle):
+    """Particles on a dyadic lattice in the cube [-1, 1]^3 sit exactly on the
+    split planes of the first levels, where the first eight bodies of a node
+    go by '<' and later ones by '<=' (src/tree.c:244-247, 282-285); with
+    bodies on the faces of the bounds, outside them, and eight coincident
+    ones.  Node array bit-identical to the oracle's serial insertion."""
+    rng = np.random.default_rng(5)
+    N = 3000
+    pos = (rng.integers(-16, 17, size=(N, 3)) / 16.0).astype(np.float32)
+    pos[:40, 0] = 1.0                      # on the upper face (inside)
+    pos[40:50, 1] = 1.5                    # outside: skipped
+    pos[50:58] = np.float32([0.3, -0.2, 0.7])   # eight coincident bodies off the lattice
+    mas = rng.uniform(0.5, 1.5, N).astype(np.float32) / N
+    bounds = np.array([-1, -1, -1, 1, 1, 1], np.float32)
+    rc, nn, raw = direct_build(pos, mas, bounds)
+    orc, _, _, onn, oraw = oracle.tree_ref(pos, mas, 0, bounds=bounds)
+    assert orc == 0 and rc == 0
+    assert nn == onn and np.array_equal(raw, oraw)
+
+
+def test_depth_and_node_limits_stop_where_the_reference_stops(oracle):
+    """Nine coincident bodies split down to depth_max; a node cap one short of
+    the full tree fails at the last split, one more node suffices.  The node
+    array of each (partial) tree equals the oracle's."""
+    pos, mas = plummer(3000, 8)
+    pos[100:109] = pos[7]
+    rc, nn, raw = direct_build(pos, mas, np.array([-4, -4, -4, 4, 4, 4], np.float32))
+    orc, _, _, onn, oraw = oracle.tree_ref(pos, mas, 0, bounds=np.array([-4, -4, -4, 4, 4, 4], np.float32))
+    assert orc == -2 and rc == -1 and nn == onn and np.array_equal(raw, oraw)   # oracle: -2 = too deep
+
+    pos, mas = plummer(3000, 9)
+    orc, bounds, _, full, _ = oracle.tree_ref(pos, mas, 3)
+    assert orc == 0
+    for cap, ok in ((full + 1, True), (full, False)):
+        rc, nn, raw = direct_build(pos, mas, bounds, max_nodes=cap)
+        orc, _, _, onn, oraw = oracle.tree_ref(pos, mas, 3, max_nodes=cap, bounds=bounds)
+        assert orc == (0 if ok else -1) and rc == orc
+        assert nn == onn and np.array_equal(raw, oraw)
+
+
+@pytest.mark.parametrize("N", [1, 8, 9, 200000])
+def test_parallel_build_and_walk_match_oracle_across_sizes(oracle, N):
+    """A single body, a full root leaf, the first split, and a 200000-body
+    Plummer sphere (about 90000 nodes): node array, accelerations and
+    potentials bit-identical to the oracle."""
+    pos, mas = plummer(N, 10 + N % 7)
+    with Simulation(N, enable_tree=1, logging=0, epsilon=1e-2, dt=1e-3) as s:
+        s.pos[:] = pos
+        s.mas[:] = mas
+        s.step_number = 4
+        s.solve()
+        rc, bounds, nodes, nn, raw = oracle.tree_ref(pos, mas, 4)
+        assert rc == 0 and s.sim.tree.num_nodes == nn
+        assert np.array_equal(s.tree_nodes(), raw)
+        acc, pot = oracle.tree_walk_ref(pos, mas, nodes, 262144, 1e-2)
+        assert np.array_equal(bits(s.acc.ravel()), bits(acc.ravel()))
+        assert np.array_equal(bits(s.pot), bits(pot))
