@@ -14,12 +14,15 @@ other two parts of the metric are measured after the timed PP steps and
 reported under "extra": the Barnes-Hut step at N = 4194304 (theta = 0.5)
 with a PP force evaluation of the same state and both force errors against
 the FP64 direct sum, and the adjoint KDK's backward/forward time ratio at
-N = 16384; plus the PP + SPH gas-ball run (N = 65536, 50 steps).
+N = 16384; plus the PP + SPH gas-ball run (N = 65536, 50 steps) and the
+reference-semantics tree solve (N = 65536), whose sim.mem is compared by
+hash with the reference's own solve timed in the CPU-baseline leg.
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
+import hashlib
 import json
 import os
 import sys
@@ -97,9 +100,69 @@ def cpu_baseline(budget_s: float = 12.0) -> dict:
     N = int(np.sqrt(rate * budget_s / 2.0))
     N = max(8192, min(N_PP, (N // 4096) * 4096))
     t = run(N, 2)
-    return {"value": N * float(N) / t / 1e9, "unit": "Ginteractions/s", "cores": threads, "kind": kind,
-            "sample": f"solve_gravity_part_part on the same Plummer workload at N={N} (2 timed evaluations, "
-                      f"-fopenmp -Og -std=c99 build of the reference sources, OMP_NUM_THREADS={threads})"}
+    out = {"value": N * float(N) / t / 1e9, "unit": "Ginteractions/s", "cores": threads, "kind": kind,
+           "sample": f"solve_gravity_part_part on the same Plummer workload at N={N} (2 timed evaluations, "
+                     f"-fopenmp -Og -std=c99 build of the reference sources, OMP_NUM_THREADS={threads})"}
+    if O.ref_available(REFTREE_VARIANT):
+        # the reference's own tree solve (sim_solv: bounds, octree, walk) on
+        # the reference-tree workload
+        pos, mas = reftree_state()
+        r = O.RefSim(REFTREE_VARIANT, REFTREE_N)
+        r.block(0, 3)[:] = pos.ravel()
+        r.block(9)[:] = mas
+        r.solve()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            r.solve()
+        t = (time.perf_counter() - t0) / 3
+        out["reference_tree_solve"] = {"N": REFTREE_N, "ms": t * 1e3,
+                                       "mem_sha256": hashlib.sha256(r.mem.tobytes()).hexdigest()}
+        r.close()
+    return out
+
+
+REFTREE_N = 65536
+REFTREE_VARIANT = "tree_e1e-2_dt1e-3"   # reference build: eps = 0.01, dt = 1e-3, theta = 0.5
+
+
+def reftree_state():
+    """Plummer sphere N = 65536 (a = 1, M = 1, seed 2): positions and masses."""
+    s = Simulation(REFTREE_N, enable_tree=0, logging=0)
+    s.ic_plummer(seed=2, first=0, count=REFTREE_N)
+    pos, mas = s.pos.copy(), s.mas.copy()
+    s.close()
+    return pos, mas
+
+
+def bench_reftree(reps: int = 5) -> dict:
+    """The reference-semantics tree solve (SF_TREE_REFERENCE: FP64 Kahan
+    bounds, octree build, walk; sim_solv) on the Plummer sphere N = 65536,
+    eps = 0.01, theta = 0.5; median of reps solves and a hash of sim.mem."""
+    pos, mas = reftree_state()
+    s = Simulation(REFTREE_N, enable_tree=1, tree_mode=L.SF_TREE_REFERENCE, logging=0, epsilon=1e-2, dt=1e-3)
+    lib = s.lib
+    s.pos[:] = pos
+    s.mas[:] = mas
+    s.solve()
+    lib.sf_prof_enable(1)
+    lib.sf_prof_reset()
+    times = []
+    for _ in range(reps):
+        hip_sync()
+        t0 = time.perf_counter()
+        s.solve()
+        hip_sync()
+        times.append(time.perf_counter() - t0)
+    build_ms, build_n = kernel_ms(lib, "tree_build_ref_par")
+    walk_ms, walk_n = kernel_ms(lib, "tree_walk_ref")
+    lib.sf_prof_enable(0)
+    out = {"workload": "reference-semantics tree solve (sim_solv), Plummer sphere N=65536 (seed 2), eps=0.01, "
+                       "theta=0.5, mirror mode",
+           "solve_ms": float(np.median(times)) * 1e3, "build_ms": build_ms / max(build_n, 1),
+           "walk_ms": walk_ms / max(walk_n, 1), "num_nodes": int(s.sim.tree.num_nodes),
+           "mem_sha256": hashlib.sha256(s.mem.tobytes()).hexdigest()}
+    s.close()
+    return out
 
 
 def rel_err_pcts(acc, ref):
@@ -334,7 +397,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the BH, adjoint and SPH measurements")
-    ap.add_argument("--only", choices=["bh", "adjoint", "sph", "copy", "bh1m", "stress"],
+    ap.add_argument("--only", choices=["bh", "adjoint", "sph", "copy", "bh1m", "stress", "reftree"],
                     help="run one secondary workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -344,7 +407,7 @@ def main():
     if args.only:
         fn = {"bh": lambda: bench_bh(steps=args.steps, pp_compare=False), "adjoint": lambda: bench_adjoint(reps=1),
               "sph": lambda: bench_sph(steps=args.steps), "copy": bench_copy, "bh1m": bench_bh1m,
-              "stress": bench_stress}[args.only]
+              "stress": bench_stress, "reftree": bench_reftree}[args.only]
         print(json.dumps(fn()))
         return
 
@@ -396,9 +459,15 @@ def main():
         "energy_check": {"K": st["K"], "W": st["W"], "T_ken": st["T"]},
     }
     if not args.no_extra:
-        out["extra"] = {"bh_4m": bench_bh(), "adjoint_16k": bench_adjoint(), "sph_65k": bench_sph()}
+        out["extra"] = {"bh_4m": bench_bh(), "adjoint_16k": bench_adjoint(), "sph_65k": bench_sph(),
+                        "reftree_65k": bench_reftree()}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
+        ref_tree = out["cpu_baseline"].get("reference_tree_solve")
+        if ref_tree and "extra" in out:
+            rt = out["extra"]["reftree_65k"]
+            rt["reference_cpu_solve_ms"] = ref_tree["ms"]
+            rt["mem_identical_to_reference"] = rt["mem_sha256"] == ref_tree["mem_sha256"]
     print(json.dumps(out))
 
 
