@@ -349,15 +349,17 @@ def bench_adjoint(reps: int = 3) -> dict:
             "grad_norm_v0": float(tv.grad.norm().item())}
 
 
-def bench_sph(steps: int = 50) -> dict:
+def bench_sph(steps: int = 50, search: int = L.SF_SPH_SEARCH_ALL) -> dict:
     """PP gravity + cubic-spline SPH: N = 65536 gas particles uniform in the
     unit ball (seeded), v = 0, m = 1/N, gamma = 5/3, u = 0.05, h = 0.05,
     eps = 0.01, dt = 5e-4, 50 KDK steps; energy E = T + W + U and momentum
-    drift."""
+    drift.  search: all pairs (the particle-particle SPH of the workload's
+    definition) or the cell grid."""
     N = 65536
+    grid = search == L.SF_SPH_SEARCH_GRID
     s = Simulation(N, enable_tree=0, enable_sph=1, sph_model=L.SF_SPH_CUBIC_IDEAL, sph_gamma=5.0 / 3.0, logging=0,
                    epsilon=1e-2, dt=5e-4, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL,
-                   mem_mode=L.SF_MEM_RESIDENT)
+                   mem_mode=L.SF_MEM_RESIDENT, sph_search=search)
     lib = s.lib
     s.ic_uniform_ball(seed=41, radius=1.0, total_mass=1.0, v_hubble=0.0)
     s.rad[:] = 0.05
@@ -375,19 +377,24 @@ def bench_sph(steps: int = 50) -> dict:
         s.step()
     hip_sync()
     step_ms = (time.perf_counter() - t0) / steps * 1e3
-    rho_ms, rho_n = kernel_ms(lib, "sph_rho_cubic")
-    frc_ms, frc_n = kernel_ms(lib, "sph_force_cubic")
+    rho_ms, rho_n = kernel_ms(lib, "sph_rho_grid" if grid else "sph_rho_cubic")
+    frc_ms, frc_n = kernel_ms(lib, "sph_force_grid" if grid else "sph_force_cubic")
+    srt_ms, srt_n = kernel_ms(lib, "sph_grid_sort")
     pp_ms, pp_n = kernel_ms(lib, "pp_fast")
     lib.sf_prof_enable(0)
     e1 = s.stats()
     s.close()
     E0 = e0["K"] + e0["W"] + e0["U"]
     E1 = e1["K"] + e1["W"] + e1["U"]
-    return {"workload": "PP gravity + PP cubic-spline SPH, uniform ball N=65536, gamma=5/3, u=0.05, h=0.05, "
-                        "eps=0.01, dt=5e-4, 50 KDK steps, fast policy, state resident",
-            "step_ms": step_ms, "sph_density_ms": rho_ms / max(rho_n, 1), "sph_force_ms": frc_ms / max(frc_n, 1),
-            "pp_force_ms": pp_ms / max(pp_n, 1), "energy_rel_drift": abs(E1 - E0) / abs(E0),
-            "momentum_abs": float(np.abs(e1["P"]).max()), "E0": E0, "E1": E1}
+    out = {"workload": "PP gravity + " + ("cell-grid" if grid else "PP") + " cubic-spline SPH, uniform ball "
+                       "N=65536, gamma=5/3, u=0.05, h=0.05, eps=0.01, dt=5e-4, 50 KDK steps, fast policy, "
+                       "state resident",
+           "step_ms": step_ms, "sph_density_ms": rho_ms / max(rho_n, 1), "sph_force_ms": frc_ms / max(frc_n, 1),
+           "pp_force_ms": pp_ms / max(pp_n, 1), "energy_rel_drift": abs(E1 - E0) / abs(E0),
+           "momentum_abs": float(np.abs(e1["P"]).max()), "E0": E0, "E1": E1}
+    if grid:
+        out["sph_grid_sort_ms"] = srt_ms / max(srt_n, 1)
+    return out
 
 
 def main():
@@ -397,7 +404,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the BH, adjoint and SPH measurements")
-    ap.add_argument("--only", choices=["bh", "adjoint", "sph", "copy", "bh1m", "stress", "reftree"],
+    ap.add_argument("--only", choices=["bh", "adjoint", "sph", "sphgrid", "copy", "bh1m", "stress", "reftree"],
                     help="run one secondary workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -406,7 +413,9 @@ def main():
         sys.exit(2)
     if args.only:
         fn = {"bh": lambda: bench_bh(steps=args.steps, pp_compare=False), "adjoint": lambda: bench_adjoint(reps=1),
-              "sph": lambda: bench_sph(steps=args.steps), "copy": bench_copy, "bh1m": bench_bh1m,
+              "sph": lambda: bench_sph(steps=args.steps),
+              "sphgrid": lambda: bench_sph(steps=args.steps, search=L.SF_SPH_SEARCH_GRID), "copy": bench_copy,
+              "bh1m": bench_bh1m,
               "stress": bench_stress, "reftree": bench_reftree}[args.only]
         print(json.dumps(fn()))
         return
@@ -460,7 +469,7 @@ def main():
     }
     if not args.no_extra:
         out["extra"] = {"bh_4m": bench_bh(), "adjoint_16k": bench_adjoint(), "sph_65k": bench_sph(),
-                        "reftree_65k": bench_reftree()}
+                        "sph_65k_grid": bench_sph(search=L.SF_SPH_SEARCH_GRID), "reftree_65k": bench_reftree()}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
         ref_tree = out["cpu_baseline"].get("reference_tree_solve")

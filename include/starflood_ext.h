@@ -66,6 +66,15 @@ enum sf_sph_model {
 	SF_SPH_CUBIC_IDEAL = 1
 };
 
+/* Pair search of the SF_SPH_CUBIC_IDEAL model. */
+enum sf_sph_search {
+	/* every pair is tested (the reference's particle-particle loops) */
+	SF_SPH_SEARCH_ALL = 0,
+	/* particles hashed into cubic cells of the largest support radius; each
+	   particle tests the 27 cells around its own */
+	SF_SPH_SEARCH_GRID = 1
+};
+
 /* Where sim.mem is authoritative between calls. */
 enum sf_mem_mode {
 	/* Every entry point uploads the blocks it reads and downloads the blocks
@@ -103,6 +112,7 @@ typedef struct {
 	int quadrupole;              /* SF_TREE_LBVH cells carry traceless quadrupoles
 	                                as well as monopoles (0: monopoles, as the
 	                                reference's tree) */
+	int sph_search;              /* enum sf_sph_search of the cubic model (GRID) */
 } sf_params_t;
 
 /* Fill *p with the defaults listed above. */

@@ -25,6 +25,7 @@ SF_PRECISION_EXACT, SF_PRECISION_FAST = 0, 1
 SF_ACC_REFERENCE, SF_ACC_PHYSICAL = 0, 1
 SF_TREE_REFERENCE, SF_TREE_LBVH = 0, 1
 SF_SPH_REFERENCE, SF_SPH_CUBIC_IDEAL = 0, 1
+SF_SPH_SEARCH_ALL, SF_SPH_SEARCH_GRID = 0, 1
 SF_MEM_MIRROR, SF_MEM_RESIDENT = 0, 1
 SF_SYNC_TO_DEVICE, SF_SYNC_TO_HOST = 0, 1
 SF_EXT_U, SF_EXT_DUDT, SF_EXT_X64 = 0, 1, 2
@@ -55,7 +56,7 @@ class Params(C.Structure):
                 ("kahan_energy", C.c_int), ("precision", C.c_int), ("acc_mode", C.c_int),
                 ("tree_mode", C.c_int), ("sph_model", C.c_int), ("mem_mode", C.c_int),
                 ("sph_gamma", C.c_double), ("logging", C.c_int), ("verbose", C.c_int),
-                ("out_dir", C.c_char * 256), ("quadrupole", C.c_int)]
+                ("out_dir", C.c_char * 256), ("quadrupole", C.c_int), ("sph_search", C.c_int)]
 
 
 class Stats(C.Structure):

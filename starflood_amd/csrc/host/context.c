@@ -94,6 +94,7 @@ void sf_ctx_destroy(sf_ctx* c) {
 	if (c->d_u) hipFree(c->d_u);
 	if (c->d_dudt) hipFree(c->d_dudt);
 	if (c->d_sph_tmp) hipFree(c->d_sph_tmp);
+	if (c->d_sph_grid) hipFree(c->d_sph_grid);
 	if (c->stream) hipStreamDestroy(c->stream);
 	free(c);
 }

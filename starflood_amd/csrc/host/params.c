@@ -38,6 +38,7 @@ void sf_params_default(sf_params_t* p) {
 	p->verbose = 0;
 	strcpy(p->out_dir, "./out");    /* config.h:75 */
 	p->quadrupole = 0;
+	p->sph_search = SF_SPH_SEARCH_GRID;
 }
 
 const sf_params_t* sf_params(void) {
@@ -72,6 +73,7 @@ int sf_set_params(const sf_params_t* p) {
 	    (p->acc_mode != SF_ACC_REFERENCE && p->acc_mode != SF_ACC_PHYSICAL) ||
 	    (p->tree_mode != SF_TREE_REFERENCE && p->tree_mode != SF_TREE_LBVH) ||
 	    (p->sph_model != SF_SPH_REFERENCE && p->sph_model != SF_SPH_CUBIC_IDEAL) ||
+	    (p->sph_search != SF_SPH_SEARCH_ALL && p->sph_search != SF_SPH_SEARCH_GRID) ||
 	    (p->mem_mode != SF_MEM_MIRROR && p->mem_mode != SF_MEM_RESIDENT)) {
 		sf_error("sf_set_params", "unknown enumeration value");
 		return STARFLOOD_FAILURE;

@@ -19,6 +19,10 @@ int sfd_sph_cubic_pack(const float* pos, const float* vel, const float* acc, con
                        hipStream_t s);
 int sfd_sph_cubic(unsigned int N, unsigned int n_pad, float gamma, int acc_phys, float* rho, float* prs, float* acc,
                   float* dudt, float* tmp, float* part, unsigned int slices, hipStream_t s);
+/* device/sph_grid.hip */
+size_t sfd_sph_grid_bytes(unsigned int N);
+int sfd_sph_cubic_grid(const float* rad, unsigned int N, unsigned int n_pad, float gamma, int acc_phys, float* rho,
+                       float* prs, float* acc, float* dudt, float* tmp, void* scratch, hipStream_t s);
 
 static int ensure_tmp(sf_ctx* c) {
 	if (c->d_sph_tmp) return 0;
@@ -44,6 +48,13 @@ int sf_solve_sph(sf_ctx* c) {
 	float* rho = sf_blk(c->d_mem, N, SF_E_RHO);
 	float* prs = sf_blk(c->d_mem, N, SF_E_PRS);
 	float* acc = sf_blk(c->d_mem, N, SF_E_ACC);
+	if (p->sph_model == SF_SPH_CUBIC_IDEAL && p->sph_search == SF_SPH_SEARCH_GRID) {
+		if (sf_ctx_ensure_u(c)) return -1;
+		if (!c->d_sph_grid) SF_HIP(hipMalloc(&c->d_sph_grid, sfd_sph_grid_bytes(N)), "solve_sph");
+		return sfd_sph_cubic_grid(sf_blk(c->d_mem, N, SF_E_RAD), N, c->n_pad, (float)p->sph_gamma,
+		                          p->acc_mode == SF_ACC_PHYSICAL, rho, prs, acc, c->d_dudt, c->d_sph_tmp,
+		                          c->d_sph_grid, c->stream);
+	}
 	if (p->sph_model == SF_SPH_CUBIC_IDEAL) {
 		/* slice partials reuse the PP kernel's scratch (same stream, after it) */
 		const unsigned int slices = sf_pp_slice_count(c->n_pad);

@@ -47,7 +47,8 @@ typedef struct sf_ctx {
 	double* h_red;           /* pinned host mirror of the reduction result */
 	float* d_u;              /* SPH specific internal energy (N) */
 	float* d_dudt;           /* its derivative (N) */
-	float* d_sph_tmp;        /* SPH scratch (4N) */
+	float* d_sph_tmp;        /* SPH scratch (packed records, 12 n_pad floats) */
+	void* d_sph_grid;        /* cell-grid pair search scratch (cubic model) */
 	hipStream_t stream;
 	sf_stats_t stats;        /* statistics of the last sim_step */
 	int have_u;

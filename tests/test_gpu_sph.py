@@ -97,15 +97,12 @@ def cubic_state(N, seed, h):
     return pos, vel, mas, rad, u
 
 
-@pytest.mark.parametrize("h", [0.15, 0.3])
-def test_cubic_model_matches_fp64_oracle(oracle, h):
+def check_cubic_against_oracle(oracle, pos, vel, mas, rad, u, search):
     """rho, P within 2e-5 relative; SPH acceleration and du/dt within 1e-4
     of the per-particle FP64 magnitude (FP32 pair sums with rsqrt)."""
-    N = 4096
-    pos, vel, mas, rad, u = cubic_state(N, 5, h)
-    rad[::7] *= 1.5                                      # mixed smoothing lengths
+    N = len(mas)
     rho_o, prs_o, acc_o, du_o = oracle.sph_cubic_f64(pos, vel, mas, rad, u, 5.0 / 3.0)
-    with cubic_sim(N, mem_mode=L.SF_MEM_MIRROR) as s:
+    with cubic_sim(N, mem_mode=L.SF_MEM_MIRROR, sph_search=search) as s:
         s.pos[:] = pos
         s.vel[:] = vel
         s.mas[:] = mas
@@ -113,12 +110,39 @@ def test_cubic_model_matches_fp64_oracle(oracle, h):
         s.acc[:] = 0
         s.set_u(u)
         s.solve_sph()
-        assert np.max(np.abs(s.rho - rho_o) / rho_o) <= 2e-5
-        assert np.max(np.abs(s.prs - prs_o) / prs_o) <= 2e-5
+        # h = 0 particles without a partner have rho = 0: relative to 1e-3 of the largest
+        assert np.max(np.abs(s.rho - rho_o) / np.maximum(rho_o, 1e-3 * rho_o.max())) <= 2e-5
+        assert np.max(np.abs(s.prs - prs_o) / np.maximum(prs_o, 1e-3 * prs_o.max())) <= 2e-5
         assert rel_vec_err(s.acc.astype(np.float64), acc_o).max() <= 1e-4
         du = s.get_dudt().astype(np.float64)
         scale = np.maximum(np.abs(du_o), 1e-3 * np.abs(du_o).max())
         assert np.max(np.abs(du - du_o) / scale) <= 1e-4
+
+
+@pytest.mark.parametrize("search", [L.SF_SPH_SEARCH_GRID, L.SF_SPH_SEARCH_ALL])
+@pytest.mark.parametrize("h", [0.05, 0.15, 0.3])
+def test_cubic_model_matches_fp64_oracle(oracle, h, search):
+    """Both pair searches (cell grid, all pairs) against the FP64 all-pairs
+    oracle, N = 4096 with mixed smoothing lengths."""
+    pos, vel, mas, rad, u = cubic_state(4096, 5, h)
+    rad[::7] *= 1.5
+    check_cubic_against_oracle(oracle, pos, vel, mas, rad, u, search)
+
+
+def test_grid_search_on_outliers_duplicates_and_zero_h(oracle):
+    """The cell grid where it is stressed: pairs of particles 2e5 and 3e5 away
+    from the ball and 0.05 apart (cell coordinates clamped to 21 bits),
+    twelve coincident particles, and particles with h = 0, which interact only
+    through their partners' h.  Same tolerances against the FP64 oracle."""
+    N = 3000
+    pos, vel, mas, rad, u = cubic_state(N, 17, 0.1)
+    pos[0] = (2.0e5, 1.0, -2.0)
+    pos[1] = (2.0e5 + 0.05, 1.0, -2.0)
+    pos[2] = (-3.0e5, -3.0e5, 5.0)
+    pos[3] = (-3.0e5, -3.0e5 + 0.04, 5.0)
+    pos[10:22] = pos[30]
+    rad[40:60] = 0.0
+    check_cubic_against_oracle(oracle, pos, vel, mas, rad, u, L.SF_SPH_SEARCH_GRID)
 
 
 def test_cubic_model_conserves_momentum_and_energy_rate():
