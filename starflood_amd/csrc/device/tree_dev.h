@@ -49,6 +49,14 @@ size_t sfd_tree_par_bytes(unsigned int N, unsigned int max_nodes);
 int sfd_tree_build_ref_par(const sf_tree_par_ctx* x, const float* pos, const float* mas, unsigned int N,
                            const float* bounds, unsigned int max_nodes, int depth_max, sf_node* node,
                            unsigned int* meta, unsigned int* order, hipStream_t s);
+/* Threaded links of a complete reference tree (meta[0] nodes; link: max_nodes
+   x 2 u32): each node's first non-empty child and the first non-empty node
+   after its subtree in the walk's order.  sfd_tree_walk_thr walks them and
+   adds the same terms in the same order as sfd_tree_walk_ref. */
+int sfd_tree_links(const sf_node* node, const unsigned int* meta, unsigned int max_nodes, void* link, hipStream_t s);
+int sfd_tree_walk_thr(const float* pos, const float* mas, unsigned int N, const unsigned int* order,
+                      const sf_node* node, const void* link, float eps2, float theta, float negG, int acc_phys,
+                      int kahan_acc, int kahan_pot, float* acc, float* pot, hipStream_t s);
 /* order: the particle of each walk lane (N entries), or NULL for index order */
 int sfd_tree_walk_ref(const float* pos, const float* mas, unsigned int N, const unsigned int* order, const sf_node* node,
                       unsigned int node_cap, unsigned int iter_max,

@@ -186,7 +186,59 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
 	s = t;
 }
 
-// Per-particle walk of src/gravity.c:273-660.
+// The opening test of src/gravity.c:334-357 for a node of more than eight
+// bodies: open iff theta <= (mean width)^2 / (distance to the centre)^2.
+__device__ __forceinline__ bool opens(const float* bmin, const float* bmax, const float* ri, float theta) {
+	const float third = (float)(1.0 / 3.0);
+	float off[3];
+	for (int a = 0; a < 3; ++a) off[a] = 0.5f * (bmin[a] + bmax[a]) - ri[a];
+	const float d2 = (off[0] * off[0] + off[1] * off[1]) + off[2] * off[2];
+	const float w = third * (((bmax[0] - bmin[0]) + (bmax[1] - bmin[1])) + (bmax[2] - bmin[2]));
+	const float ratio = 0.0f < d2 ? (w * w) / d2 : 0.0f;
+	return theta <= ratio;
+}
+
+// The node as one monopole, less particle i when the node's box holds it
+// (src/gravity.c:380-640), added into the (Kahan) sums.
+template <bool KAHAN_ACC, bool KAHAN_POT>
+__device__ __forceinline__ void add_monopole(const sf_node* __restrict__ nd, unsigned bodies, const float* bmin,
+                                             const float* bmax, const float* ri, float mi, float eps2, float* sum,
+                                             float* com) {
+	float rj[3] = {nd->params[0], nd->params[1], nd->params[2]};
+	float mj = nd->params[3];
+	if (bmin[0] <= ri[0] && ri[0] <= bmax[0] && bmin[1] <= ri[1] && ri[1] <= bmax[1] && bmin[2] <= ri[2] &&
+	    ri[2] <= bmax[2] && 1u < bodies) {
+		for (int a = 0; a < 3; ++a) rj[a] = rj[a] * mj;
+		for (int a = 0; a < 3; ++a) rj[a] = rj[a] - mi * ri[a];
+		mj = mj - mi;
+		for (int a = 0; a < 3; ++a) rj[a] = rj[a] * (0.0f < mj ? 1.0f / mj : 1.0f);
+	}
+	const float rx = ri[0] - rj[0], ry = ri[1] - rj[1], rz = ri[2] - rj[2];
+	const float r2 = (rx * rx + ry * ry) + rz * rz;
+	const float inv_r2 = 1.0f / (r2 + eps2);
+	const float inv_r1 = 1.0f / sqrtf(r2 + eps2);
+	const float pj = mj * inv_r1;
+	const float f[3] = {pj * inv_r2 * rx, pj * inv_r2 * ry, pj * inv_r2 * rz};
+	for (int a = 0; a < 3; ++a) {
+		if (KAHAN_ACC) kahan_add(sum[a], com[a], f[a]);
+		else sum[a] += f[a];
+	}
+	if (KAHAN_POT) kahan_add(sum[3], com[3], pj);
+	else sum[3] += pj;
+}
+
+template <bool PHYS>
+__device__ __forceinline__ void store_walk(unsigned idx, float mi, float negG, const float* sum,
+                                           float* __restrict__ acc, float* __restrict__ pot) {
+	const float f = PHYS ? negG : negG * mi;
+	acc[3u * idx + 0u] = f * sum[0];
+	acc[3u * idx + 1u] = f * sum[1];
+	acc[3u * idx + 2u] = f * sum[2];
+	pot[idx] = negG * sum[3];
+}
+
+// Per-particle walk of src/gravity.c:273-660: the reference's parent-pointer
+// walk with its path stack and iteration cap.
 template <bool KAHAN_ACC, bool KAHAN_POT, bool PHYS>
 __global__ __launch_bounds__(kBlock) void k_tree_walk_ref(const float* __restrict__ pos, const float* __restrict__ mas,
                                                           unsigned N, const unsigned* __restrict__ order,
@@ -197,15 +249,13 @@ __global__ __launch_bounds__(kBlock) void k_tree_walk_ref(const float* __restric
 	const unsigned lane_id = blockIdx.x * kBlock + threadIdx.x;
 	if (lane_id >= N) return;
 	const unsigned idx = order ? order[lane_id] : lane_id;
-	float sx = 0.0f, sy = 0.0f, sz = 0.0f, sp = 0.0f;
-	float cx = 0.0f, cy = 0.0f, cz = 0.0f, cp = 0.0f;
+	float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f}, com[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 	const float ri[3] = {pos[3u * idx], pos[3u * idx + 1u], pos[3u * idx + 2u]};
 	const float mi = mas[idx];
 	unsigned next_node = 0u;
 	int cur_depth = 0;
 	int path[kDepthCap];
 	for (int k = 0; k < kDepthCap; ++k) path[k] = -1;
-	const float third = (float)(1.0 / 3.0);
 
 	for (unsigned it = 0; it < iter_max; ++it) {
 		if (cur_depth < 0) break;
@@ -233,12 +283,7 @@ __global__ __launch_bounds__(kBlock) void k_tree_walk_ref(const float* __restric
 				continue;
 			}
 			if (octant < 0) {
-				float off[3];
-				for (int a = 0; a < 3; ++a) off[a] = 0.5f * (bmin[a] + bmax[a]) - ri[a];
-				const float d2 = (off[0] * off[0] + off[1] * off[1]) + off[2] * off[2];
-				const float w = third * (((bmax[0] - bmin[0]) + (bmax[1] - bmin[1])) + (bmax[2] - bmin[2]));
-				const float ratio = 0.0f < d2 ? (w * w) / d2 : 0.0f;
-				if (theta <= ratio) {
+				if (opens(bmin, bmax, ri, theta)) {
 					next_node = nd->branch[0];
 					path[cur_depth] = 0;
 					cur_depth += 1;
@@ -251,43 +296,83 @@ __global__ __launch_bounds__(kBlock) void k_tree_walk_ref(const float* __restric
 				continue;
 			}
 		}
-		// the node as one monopole
-		float rj[3] = {nd->params[0], nd->params[1], nd->params[2]};
-		float mj = nd->params[3];
-		if (bmin[0] <= ri[0] && ri[0] <= bmax[0] && bmin[1] <= ri[1] && ri[1] <= bmax[1] && bmin[2] <= ri[2] &&
-		    ri[2] <= bmax[2] && 1u < bodies) {
-			for (int a = 0; a < 3; ++a) rj[a] = rj[a] * mj;
-			for (int a = 0; a < 3; ++a) rj[a] = rj[a] - mi * ri[a];
-			mj = mj - mi;
-			for (int a = 0; a < 3; ++a) rj[a] = rj[a] * (0.0f < mj ? 1.0f / mj : 1.0f);
-		}
-		const float rx = ri[0] - rj[0], ry = ri[1] - rj[1], rz = ri[2] - rj[2];
-		const float r2 = (rx * rx + ry * ry) + rz * rz;
-		const float inv_r2 = 1.0f / (r2 + eps2);
-		const float inv_r1 = 1.0f / sqrtf(r2 + eps2);
-		const float pj = mj * inv_r1;
-		const float fx = pj * inv_r2 * rx;
-		const float fy = pj * inv_r2 * ry;
-		const float fz = pj * inv_r2 * rz;
-		if (KAHAN_ACC) {
-			kahan_add(sx, cx, fx);
-			kahan_add(sy, cy, fy);
-			kahan_add(sz, cz, fz);
-		} else {
-			sx += fx;
-			sy += fy;
-			sz += fz;
-		}
-		if (KAHAN_POT) kahan_add(sp, cp, pj);
-		else sp += pj;
+		add_monopole<KAHAN_ACC, KAHAN_POT>(nd, bodies, bmin, bmax, ri, mi, eps2, sum, com);
 		next_node = parent;
 		cur_depth -= 1;
 	}
-	const float f = PHYS ? negG : negG * mi;
-	acc[3u * idx + 0u] = f * sx;
-	acc[3u * idx + 1u] = f * sy;
-	acc[3u * idx + 2u] = f * sz;
-	pot[idx] = negG * sp;
+	store_walk<PHYS>(idx, mi, negG, sum, acc, pot);
+}
+
+// The same walk over threaded links: down[n] is n's first non-empty child,
+// next[n] the first non-empty node after n's subtree in the walk's order
+// (children in octant order), kEnd past the last.  It visits the reference
+// walk's non-empty nodes in its order without the returns to the parent;
+// empty nodes add nothing there.  For a complete tree the reference walk
+// takes 16 (opened nodes) + 1 <= 2 num_nodes - 1 iterations, below its cap,
+// so the two walks add the same terms in the same order.
+constexpr unsigned kEnd = 0xFFFFFFFFu;
+
+template <bool KAHAN_ACC, bool KAHAN_POT, bool PHYS>
+__global__ __launch_bounds__(kBlock) void k_tree_walk_thr(const float* __restrict__ pos, const float* __restrict__ mas,
+                                                          unsigned N, const unsigned* __restrict__ order,
+                                                          const sf_node* __restrict__ node,
+                                                          const uint2* __restrict__ link, float eps2, float theta,
+                                                          float negG, float* __restrict__ acc,
+                                                          float* __restrict__ pot) {
+	const unsigned lane_id = blockIdx.x * kBlock + threadIdx.x;
+	if (lane_id >= N) return;
+	const unsigned idx = order ? order[lane_id] : lane_id;
+	float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f}, com[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+	const float ri[3] = {pos[3u * idx], pos[3u * idx + 1u], pos[3u * idx + 2u]};
+	const float mi = mas[idx];
+	unsigned n = node[0].bodies ? 0u : kEnd;
+	while (n != kEnd) {
+		const sf_node* nd = &node[n];
+		const uint2 ln = link[n];   // (down, next)
+		const unsigned bodies = nd->bodies;
+		const float bmin[3] = {nd->bounds[0], nd->bounds[1], nd->bounds[2]};
+		const float bmax[3] = {nd->bounds[3], nd->bounds[4], nd->bounds[5]};
+		if (bodies > 8u && opens(bmin, bmax, ri, theta)) {
+			n = ln.x;
+			continue;
+		}
+		add_monopole<KAHAN_ACC, KAHAN_POT>(nd, bodies, bmin, bmax, ri, mi, eps2, sum, com);
+		n = ln.y;
+	}
+	store_walk<PHYS>(idx, mi, negG, sum, acc, pot);
+}
+
+// link[t] = (first non-empty child, first non-empty node after t's subtree)
+// for the num_nodes nodes of a complete tree.  Walking up through last
+// children takes at most the tree depth (16) steps.
+__global__ __launch_bounds__(kBlock) void k_tree_links(const sf_node* __restrict__ node, const unsigned* __restrict__ meta,
+                                                       uint2* __restrict__ link) {
+	const unsigned t = blockIdx.x * kBlock + threadIdx.x;
+	const unsigned num_nodes = meta[0];
+	if (t >= num_nodes) return;
+	unsigned down = kEnd;
+	if (node[t].bodies > 8u)
+		for (int k = 7; k >= 0; --k) {
+			const unsigned c = node[t].branch[k];
+			if (node[c].bodies) down = c;
+		}
+	unsigned next = kEnd;
+	unsigned n = t;
+	for (int up = 0; up <= kDepthCap && n != 0u; ++up) {
+		const unsigned par = node[n].parent;
+		int k = 0;
+		while (k < 8 && node[par].branch[k] != n) ++k;
+		for (++k; k < 8; ++k) {
+			const unsigned c = node[par].branch[k];
+			if (node[c].bodies) {
+				next = c;
+				break;
+			}
+		}
+		if (next != kEnd) break;
+		n = par;
+	}
+	link[t] = make_uint2(down, next);
 }
 
 }  // namespace
@@ -335,4 +420,38 @@ extern "C" int sfd_tree_walk_ref(const float* pos, const float* mas, unsigned N,
 #undef SF_WALK
 	sf_prof_end(tok, s);
 	return sf_hip_check(hipGetLastError(), "sfd_tree_walk_ref", "walk launch");
+}
+
+extern "C" int sfd_tree_links(const sf_node* node, const unsigned* meta, unsigned max_nodes, void* link,
+                              hipStream_t s) {
+	const unsigned grid = (max_nodes + kBlock - 1) / kBlock;
+	hipLaunchKernelGGL(k_tree_links, dim3(grid ? grid : 1), dim3(kBlock), 0, s, node, meta,
+	                   static_cast<uint2*>(link));
+	return sf_hip_check(hipGetLastError(), "sfd_tree_links", "launch");
+}
+
+extern "C" int sfd_tree_walk_thr(const float* pos, const float* mas, unsigned N, const unsigned* order,
+                                 const sf_node* node, const void* link, float eps2, float theta, float negG,
+                                 int acc_phys, int kahan_acc, int kahan_pot, float* acc, float* pot, hipStream_t s) {
+	if (N == 0) return 0;
+	const dim3 grid((N + kBlock - 1) / kBlock), block(kBlock);
+	const uint2* ln = static_cast<const uint2*>(link);
+	const int sel = (kahan_acc ? 4 : 0) | (kahan_pot ? 2 : 0) | (acc_phys ? 1 : 0);
+	const int tok = sf_prof_begin("tree_walk_ref", s);
+#define SF_WALK(A, B, C)                                                                                         \
+	hipLaunchKernelGGL((k_tree_walk_thr<A, B, C>), grid, block, 0, s, pos, mas, N, order, node, ln, eps2, theta, negG, \
+	                   acc, pot)
+	switch (sel) {
+	case 0: SF_WALK(false, false, false); break;
+	case 1: SF_WALK(false, false, true); break;
+	case 2: SF_WALK(false, true, false); break;
+	case 3: SF_WALK(false, true, true); break;
+	case 4: SF_WALK(true, false, false); break;
+	case 5: SF_WALK(true, false, true); break;
+	case 6: SF_WALK(true, true, false); break;
+	default: SF_WALK(true, true, true); break;
+	}
+#undef SF_WALK
+	sf_prof_end(tok, s);
+	return sf_hip_check(hipGetLastError(), "sfd_tree_walk_thr", "walk launch");
 }

@@ -18,10 +18,11 @@
 // sorted with the stable radix sort; a slot's members are counted, their
 // mass-moment products stored in that order, and the members of splitting
 // slots routed by rank to their child slot.  Nodes are numbered level-major
-// while building.  Every node's moments are then summed in index order,
-// sequentially as the reference adds them (the root's chain, the longest,
-// on a second stream while the deeper levels are built); the split events,
-// sorted by (i9, depth), map the level-major numbering onto the reference's.
+// while building.  Every node's moments are summed in index order,
+// sequentially as the reference adds them, on a second stream as soon as the
+// node's level is finished (the root's chain, the longest, beside the
+// deeper levels); the split events, sorted by (i9, depth), map the
+// level-major numbering onto the reference's.
 //
 // The level sizes live on the device (st[]), so the host queues several
 // levels at a time and synchronises once per batch.  A build that would run
@@ -433,6 +434,17 @@ bool par_applies(unsigned N, unsigned max_nodes) {
 	       (unsigned long long)kMaxLevels * N < (1ull << 32);
 }
 
+// Moment sums of nodes [first, first + count): lanes for the small ones,
+// wavefronts for the others.
+void launch_sums(const float* arch, size_t a_cap, const uint2* trange, unsigned first, unsigned count, sf_node* tnode,
+                 hipStream_t s) {
+	const unsigned items = 4u * count;
+	hipLaunchKernelGGL(k_node_sums_small, dim3(blocks(items)), dim3(kBlock), 0, s, arch, a_cap, trange, first, count,
+	                   tnode);
+	hipLaunchKernelGGL(k_node_sums, dim3(items < 4096u ? items : 4096u), dim3(64), 0, s, arch, a_cap, trange, first,
+	                   count, kSumSmall, tnode);
+}
+
 int build_par(const sf_tree_par_ctx* x, const float* pos, const float* mas, unsigned N, const float* bounds,
               unsigned max_nodes, int depth_max, sf_node* node, unsigned* meta, unsigned* order, hipStream_t s) {
 	if (!par_applies(N, max_nodes) || depth_max < 0 || depth_max >= (int)kMaxLevels) return 1;
@@ -465,6 +477,7 @@ int build_par(const sf_tree_par_ctx* x, const float* pos, const float* mas, unsi
 	const unsigned passes = passes_for(8ull * (N / 9u));
 	const unsigned n_pad = round_up(N ? N : 1u, sfd_radix_tile());
 	unsigned levels = 0;
+	unsigned summed = 1u;   // nodes [1, summed) have their sums queued on the side stream
 	while (levels < kMaxLevels) {
 		const unsigned batch = kMaxLevels - levels < kLevelBatch ? kMaxLevels - levels : kLevelBatch;
 		for (unsigned b = 0; b < batch; ++b, ++levels) {
@@ -494,22 +507,24 @@ int build_par(const sf_tree_par_ctx* x, const float* pos, const float* mas, unsi
 		    sf_hip_check(hipStreamSynchronize(s), "sfd_tree_build_ref_par", "levels"))
 			return -1;
 		if (x->h_state[ST_STATUS] != RUNNING) break;
+		// nodes of the finished levels (all before the current level's first
+		// slot) are complete: their sums run beside the next levels
+		const unsigned upto = x->h_state[ST_BASE];
+		if (upto > summed) {
+			launch_sums(arch, L.a_cap, trange, summed, upto - summed, tnode, x->side);
+			if (sf_hip_check(hipEventRecord(x->ev[1], x->side), "sfd_tree_build_ref_par", "event")) return -1;
+			summed = upto;
+		}
 	}
 	if (x->h_state[ST_STATUS] != DONE) {
-		// the root's sums may still read the archive: finish them first
+		// sums on the side stream may still read the archive: finish them first
 		if (sf_hip_check(hipStreamSynchronize(x->side), "sfd_tree_build_ref_par", "side")) return -1;
 		return 1;
 	}
 
 	const unsigned S = x->h_state[ST_EVOFF];
 	const unsigned num_nodes = 1u + 8u * S;
-	if (num_nodes > 1u) {
-		const unsigned items = 4u * (num_nodes - 1u);
-		hipLaunchKernelGGL(k_node_sums_small, dim3(blocks(items)), dim3(kBlock), 0, s, arch, L.a_cap, trange, 1u,
-		                   num_nodes - 1u, tnode);
-		hipLaunchKernelGGL(k_node_sums, dim3(items < 4096u ? items : 4096u), dim3(64), 0, s, arch, L.a_cap, trange, 1u,
-		                   num_nodes - 1u, kSumSmall, tnode);
-	}
+	if (num_nodes > summed) launch_sums(arch, L.a_cap, trange, summed, num_nodes - summed, tnode, s);
 	if (S) {
 		const unsigned ev_pad = round_up(S, sfd_radix_tile());
 		if (ev_pad > S)

@@ -36,6 +36,7 @@ struct sf_tree_dev {
 	unsigned int r1_n;
 	unsigned int* d_order;   /* walk order of the last parallel build (N entries) */
 	unsigned int order_n;    /* its N, 0 when the last build was serial */
+	void* d_link;            /* threaded links of the last complete parallel build (max_nodes x 2 u32) */
 	sf_lbvh* lbvh;           /* SF_TREE_LBVH state (lbvh.c) */
 };
 
@@ -69,6 +70,7 @@ static void tree_dev_destroy(sf_tree_dev* d) {
 	if (d->par.side) hipStreamDestroy(d->par.side);
 	if (d->d_r1) hipFree(d->d_r1);
 	if (d->d_order) hipFree(d->d_order);
+	if (d->d_link) hipFree(d->d_link);
 	if (d->stream) hipStreamDestroy(d->stream);
 	free(d);
 }
@@ -174,7 +176,11 @@ static int build_ref(sf_tree_dev* d, tree_t* t, unsigned int N, const float* pos
 	const int rc = need ? sfd_tree_build_ref_par(&d->par, pos, mas, N, d->d_bounds, d->max_nodes,
 	                                             (int)p->tree_depth_max, d->d_node, d->d_meta, d->d_order, s)
 	                    : 1;
-	if (rc == 0) d->order_n = N;
+	if (rc == 0) {
+		if (!d->d_link) SF_HIP(hipMalloc(&d->d_link, (size_t)d->max_nodes * 2u * sizeof(unsigned int)), "tree_build");
+		if (sfd_tree_links(d->d_node, d->d_meta, d->max_nodes, d->d_link, s)) return -1;
+		d->order_n = N;
+	}
 	if (rc < 0) return -1;
 	if (rc > 0 &&
 	    sfd_tree_build_ref(pos, mas, N, d->d_bounds, d->max_nodes, (int)p->tree_depth_max, d->d_node, d->d_meta, s))
@@ -236,6 +242,21 @@ int tree_build(tree_t* SF_RESTRICT tree_ptr, unsigned int num_parts, const real*
 	return rc ? STARFLOOD_FAILURE : STARFLOOD_SUCCESS;
 }
 
+/* Walk of the reference tree last built (src/gravity.c:245-663): over the
+   threaded links after a complete parallel build, else the reference's own
+   parent-pointer walk (which also walks a partial tree as the reference
+   does). */
+static int walk_ref(sf_tree_dev* d, unsigned int N, const float* pos, const float* mas, float eps2, float* acc,
+                    float* pot, hipStream_t s) {
+	const sf_params_t* p = sf_params();
+	const int phys = p->acc_mode == SF_ACC_PHYSICAL;
+	if (d->order_n == N && N)
+		return sfd_tree_walk_thr(pos, mas, N, d->d_order, d->d_node, d->d_link, eps2, (float)p->theta, (float)(-p->G),
+		                         phys, p->kahan_solver, p->kahan_energy, acc, pot, s);
+	return sfd_tree_walk_ref(pos, mas, N, NULL, d->d_node, d->max_nodes, 2u * 8u * p->tree_nodes_max, eps2,
+	                         (float)p->theta, (float)(-p->G), phys, p->kahan_solver, p->kahan_energy, acc, pot, s);
+}
+
 static void bounds_jitter(unsigned int step_number, double jitter[4]) {
 	/* src/simulation.c:281-296 */
 	uint32_t s[4] = {0x638FE17Bu + (uint32_t)step_number, 0xB10BDCE7u, 0x7B8ADB9Bu, 0xD8C5850Eu};
@@ -254,7 +275,6 @@ int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number) {
 	const float* mas = sf_blk(c->d_mem, N, SF_E_MAS);
 	float* acc = sf_blk(c->d_mem, N, SF_E_ACC);
 	float* pot = sf_blk(c->d_mem, N, SF_E_POT);
-	const int phys = p->acc_mode == SF_ACC_PHYSICAL;
 	const float eps2 = (float)(p->epsilon * p->epsilon);
 	if (p->tree_mode == SF_TREE_REFERENCE) {
 		double jit[4];
@@ -272,9 +292,7 @@ int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number) {
 		SF_HIP(hipMemcpyAsync(tree->bounds, d->d_bounds, 6u * sizeof(float), hipMemcpyDeviceToHost, s),
 		       "solve_gravity_part_tree");
 		/* the reference walks the tree even after a failed build */
-		if (sfd_tree_walk_ref(pos, mas, N, d->order_n == N ? d->d_order : NULL, d->d_node, d->max_nodes, 2u * 8u * p->tree_nodes_max, eps2, (float)p->theta,
-		                      (float)(-p->G), phys, p->kahan_solver, p->kahan_energy, acc, pot, s))
-			return -1;
+		if (walk_ref(d, N, pos, mas, eps2, acc, pot, s)) return -1;
 		return rc;
 	}
 	if (sf_lbvh_build(&d->lbvh, N, pos, mas, p->quadrupole, s)) return -1;
@@ -329,9 +347,7 @@ int solve_gravity_part_tree(const sim_t* SF_RESTRICT sim_ptr) {
 	float* pot = sf_blk(c->d_mem, N, SF_E_POT);
 	int rc;
 	if (p->tree_mode == SF_TREE_REFERENCE) {
-		rc = sfd_tree_walk_ref(pos, mas, N, d->order_n == N ? d->d_order : NULL, d->d_node, d->max_nodes, 2u * 8u * p->tree_nodes_max, (float)(p->epsilon * p->epsilon),
-		                       (float)p->theta, (float)(-p->G), p->acc_mode == SF_ACC_PHYSICAL, p->kahan_solver,
-		                       p->kahan_energy, acc, pot, s);
+		rc = walk_ref(d, N, pos, mas, (float)(p->epsilon * p->epsilon), acc, pot, s);
 	} else {
 		rc = d->lbvh ? sf_lbvh_walk(d->lbvh, N, pos, mas, p, acc, pot, s) : -1;
 		if (!d->lbvh) sf_error("solve_gravity_part_tree", "no tree was built");
