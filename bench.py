@@ -118,6 +118,14 @@ def cpu_baseline(budget_s: float = 12.0) -> dict:
         out["reference_tree_solve"] = {"N": REFTREE_N, "ms": t * 1e3,
                                        "mem_sha256": hashlib.sha256(r.mem.tobytes()).hexdigest()}
         r.close()
+    if O.ref_available("pp_e1e-2_dt1e-3"):
+        # the reference's own vis_draw of the same frame
+        r = O.RefSim("pp_e1e-2_dt1e-3", VIS_N)
+        r.mem[:] = vis_state()
+        r.sim.step_number = 10
+        out["reference_vis_draw"] = {"N": VIS_N, "size": VIS_SIZE,
+                                     "ms": r.vis_draw_seconds(VIS_SIZE, VIS_SIZE, reps=3) * 1e3}
+        r.close()
     return out
 
 
@@ -161,6 +169,52 @@ def bench_reftree(reps: int = 5) -> dict:
            "solve_ms": float(np.median(times)) * 1e3, "build_ms": build_ms / max(build_n, 1),
            "walk_ms": walk_ms / max(walk_n, 1), "num_nodes": int(s.sim.tree.num_nodes),
            "mem_sha256": hashlib.sha256(s.mem.tobytes()).hexdigest()}
+    s.close()
+    return out
+
+
+VIS_N = 65536
+VIS_SIZE = 1080   # the reference driver's frame (src/main.c:81)
+
+
+def vis_state():
+    """Plummer sphere N = 65536 (seed 3) after one PP solve (eps = 0.01)."""
+    s = Simulation(VIS_N, enable_tree=0, logging=0, epsilon=1e-2, dt=1e-3, precision=L.SF_PRECISION_FAST)
+    s.ic_plummer(seed=3, first=0, count=VIS_N)
+    s.solve()
+    mem = s.mem.copy()
+    s.close()
+    return mem
+
+
+def bench_vis(reps: int = 5) -> dict:
+    """vis_draw (128 samples per particle, integer splat, tanh tone map) of
+    the Plummer sphere N = 65536 into the 1080 x 1080 frame of the
+    reference driver; the median of reps draws and the device time."""
+    from starflood_amd.vis import Visualization
+    mem = vis_state()
+    s = Simulation(VIS_N, enable_tree=0, logging=0, epsilon=1e-2, dt=1e-3)
+    lib = s.lib
+    s.mem[:] = mem
+    s.step_number = 10
+    v = Visualization(VIS_SIZE, VIS_SIZE)
+    v.draw(s)
+    lib.sf_prof_enable(1)
+    lib.sf_prof_reset()
+    times = []
+    for _ in range(reps):
+        hip_sync()
+        t0 = time.perf_counter()
+        v.draw(s)
+        times.append(time.perf_counter() - t0)
+    dev_ms, dev_n = kernel_ms(lib, "vis_draw")
+    lib.sf_prof_enable(0)
+    out = {"workload": "vis_draw, Plummer sphere N=65536 (seed 3) after a PP solve, 1080x1080, 128 samples per "
+                       "particle, mirror mode (buffers copied back to the host)",
+           "draw_ms": float(np.median(times)) * 1e3, "device_ms": dev_ms / max(dev_n, 1),
+           "samples_per_s": VIS_N * 128 / (dev_ms / max(dev_n, 1) / 1e3),
+           "samples_in_frame": int(v.atomic[..., 3].sum())}
+    v.close()
     s.close()
     return out
 
@@ -404,7 +458,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the BH, adjoint and SPH measurements")
-    ap.add_argument("--only", choices=["bh", "adjoint", "sph", "sphgrid", "copy", "bh1m", "stress", "reftree"],
+    ap.add_argument("--only", choices=["bh", "adjoint", "sph", "sphgrid", "copy", "bh1m", "stress", "reftree",
+                                       "vis"],
                     help="run one secondary workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -416,7 +471,7 @@ def main():
               "sph": lambda: bench_sph(steps=args.steps),
               "sphgrid": lambda: bench_sph(steps=args.steps, search=L.SF_SPH_SEARCH_GRID), "copy": bench_copy,
               "bh1m": bench_bh1m,
-              "stress": bench_stress, "reftree": bench_reftree}[args.only]
+              "stress": bench_stress, "reftree": bench_reftree, "vis": bench_vis}[args.only]
         print(json.dumps(fn()))
         return
 
@@ -469,7 +524,8 @@ def main():
     }
     if not args.no_extra:
         out["extra"] = {"bh_4m": bench_bh(), "adjoint_16k": bench_adjoint(), "sph_65k": bench_sph(),
-                        "sph_65k_grid": bench_sph(search=L.SF_SPH_SEARCH_GRID), "reftree_65k": bench_reftree()}
+                        "sph_65k_grid": bench_sph(search=L.SF_SPH_SEARCH_GRID), "reftree_65k": bench_reftree(),
+                        "vis_65k": bench_vis()}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
         ref_tree = out["cpu_baseline"].get("reference_tree_solve")
@@ -477,6 +533,9 @@ def main():
             rt = out["extra"]["reftree_65k"]
             rt["reference_cpu_solve_ms"] = ref_tree["ms"]
             rt["mem_identical_to_reference"] = rt["mem_sha256"] == ref_tree["mem_sha256"]
+        ref_vis = out["cpu_baseline"].get("reference_vis_draw")
+        if ref_vis and "extra" in out:
+            out["extra"]["vis_65k"]["reference_cpu_draw_ms"] = ref_vis["ms"]
     print(json.dumps(out))
 
 

@@ -15,6 +15,7 @@ import ctypes as C
 import os
 import subprocess
 import tempfile
+import time
 from contextlib import contextmanager
 from pathlib import Path
 
@@ -268,6 +269,32 @@ class _RefSimTree(C.Structure):
     _fields_ = [("step_number", C.c_uint), ("N", C.c_uint), ("mem", C.c_void_p), ("tree", _RefTree)]
 
 
+def ref_vis_save(render, filetype: int, path=None):
+    """The reference build's vis_save of a (h, w, 4) float render buffer:
+    the 8-bit pixels it converts for PPM and PNG (PNG: the conversion only,
+    the build has no PNG writer), and the file it writes when path is
+    given.  Not PFM: in this (PNG-format) build the binary buffer holds
+    3 bytes per pixel and the PFM path writes 12 (src/visualization.c:76-87,
+    src/visualization_io.c:207-216), past the end of the allocation."""
+    if filetype == 0:
+        raise ValueError("the reference's PFM path overflows its buffer in this build")
+    lib = C.CDLL(str(REF_DIR / "libref_pp_e1e-2_dt1e-3.so"))
+    render = np.ascontiguousarray(render, np.float32)
+    h, w = render.shape[:2]
+    vis = VisT()
+    with tempfile.TemporaryDirectory() as d, _in_dir(d), capture_stdout():
+        os.makedirs("out", exist_ok=True)
+        assert lib.vis_init(C.byref(vis), C.c_uint(w), C.c_uint(h)) == 0
+        try:
+            C.memmove(vis.render_buffer, render.ctypes.data, render.nbytes)
+            target = str(path) if path is not None else os.path.join(d, "unused")
+            assert lib.vis_save(C.byref(vis), target.encode(), C.c_int(filetype)) == 0
+            pixels = np.frombuffer((C.c_uint8 * (3 * w * h)).from_address(vis.binary_buffer), np.uint8).copy()
+        finally:
+            lib.vis_free(C.byref(vis))
+    return pixels
+
+
 def ref_available(variant: str) -> bool:
     return (REF_DIR / f"libref_{variant}.so").exists()
 
@@ -299,6 +326,12 @@ def capture_stdout():
             os.close(saved)
             tmp.seek(0)
             out.append(tmp.read().decode(errors="replace"))
+
+
+class VisT(C.Structure):
+    """vis_t, reference src/visualization.h:19-27."""
+    _fields_ = [("sizex", C.c_uint), ("sizey", C.c_uint), ("mem", C.c_void_p), ("atomic_buffer", C.c_void_p),
+                ("render_buffer", C.c_void_p), ("binary_buffer", C.c_void_p)]
 
 
 class RefSim:
@@ -351,6 +384,48 @@ class RefSim:
     def statistics(self) -> str:
         with open(os.path.join(self.tmp.name, "out", "statistics.csv")) as f:
             return f.read()
+
+    def vis_frame(self, sizex: int, sizey: int, files=()):
+        """vis_init, vis_draw of the current state and step number, vis_save
+        to each (path, image_filetype_t) of files, vis_free: the atomic and
+        render buffers (sizey, sizex, 4) and the reference's matMVP."""
+        vis = VisT()
+        for fn in ("vis_init", "vis_free", "vis_draw", "vis_save"):
+            getattr(self.lib, fn).restype = C.c_int
+        with _in_dir(self.tmp.name), capture_stdout():
+            assert self.lib.vis_init(C.byref(vis), C.c_uint(sizex), C.c_uint(sizey)) == 0
+            try:
+                assert self.lib.vis_draw(C.byref(vis), C.byref(self.sim)) == 0
+                n = 4 * sizex * sizey
+                atomic = np.frombuffer((C.c_int32 * n).from_address(vis.atomic_buffer), np.int32).copy()
+                render = np.frombuffer((C.c_float * n).from_address(vis.render_buffer), np.float32).copy()
+                mvp = np.frombuffer((C.c_float * 16).in_dll(self.lib, "matMVP"), np.float32).copy()
+                for path, ftype in files:
+                    assert self.lib.vis_save(C.byref(vis), str(path).encode(), C.c_int(ftype)) == 0
+                # the 8-bit pixels of the last PPM or PNG save (written even
+                # where the build has no PNG writer)
+                pixels = np.frombuffer((C.c_uint8 * (3 * sizex * sizey)).from_address(vis.binary_buffer),
+                                       np.uint8).copy()
+            finally:
+                self.lib.vis_free(C.byref(vis))   # joins the image-writer thread
+        return atomic.reshape(sizey, sizex, 4), render.reshape(sizey, sizex, 4), mvp, pixels
+
+    def vis_draw_seconds(self, sizex: int, sizey: int, reps: int = 3) -> float:
+        """Mean wall time of the reference's vis_draw of the current state."""
+        vis = VisT()
+        for fn in ("vis_init", "vis_free", "vis_draw"):
+            getattr(self.lib, fn).restype = C.c_int
+        with _in_dir(self.tmp.name), capture_stdout():
+            assert self.lib.vis_init(C.byref(vis), C.c_uint(sizex), C.c_uint(sizey)) == 0
+            try:
+                assert self.lib.vis_draw(C.byref(vis), C.byref(self.sim)) == 0
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    assert self.lib.vis_draw(C.byref(vis), C.byref(self.sim)) == 0
+                dt = (time.perf_counter() - t0) / reps
+            finally:
+                self.lib.vis_free(C.byref(vis))
+        return dt
 
     def tree_nodes(self, num_nodes: int) -> np.ndarray:
         raw = (C.c_uint8 * (num_nodes * 80)).from_address(self.sim.tree.node)

@@ -22,6 +22,10 @@
 #undef ENABLE_OFFLOAD_VIS
 #undef ENABLE_TIMING
 #undef ENABLE_VIS
+/* the visualisation is built without its stb PNG writer (absent here; PFM
+   and PPM stay) and without its timing log */
+#undef ENABLE_STB_IMAGE_WRITE
+#undef LOG_TIMINGS_VIS_DRAW
 
 #undef EPSILON
 #define EPSILON SFREF_EPSILON

@@ -59,6 +59,15 @@ class Params(C.Structure):
                 ("out_dir", C.c_char * 256), ("quadrupole", C.c_int), ("sph_search", C.c_int)]
 
 
+class VisT(C.Structure):
+    """vis_t, reference src/visualization.h:18-27."""
+    _fields_ = [("sizex", C.c_uint), ("sizey", C.c_uint), ("mem", C.c_void_p), ("atomic_buffer", C.c_void_p),
+                ("render_buffer", C.c_void_p), ("binary_buffer", C.c_void_p)]
+
+
+IMAGE_FILETYPE_PFM, IMAGE_FILETYPE_PPM, IMAGE_FILETYPE_PNG = 0, 1, 2
+
+
 class Stats(C.Structure):
     """sf_stats_t."""
     _fields_ = [("T", C.c_double), ("V", C.c_double), ("E", C.c_double), ("W", C.c_double), ("U", C.c_double),
@@ -67,6 +76,10 @@ class Stats(C.Structure):
 
 # (name, restype, argtypes) for every exported entry point
 _SIG = [
+    ("vis_init", C.c_int, [C.POINTER(VisT), C.c_uint, C.c_uint]),
+    ("vis_free", C.c_int, [C.POINTER(VisT)]),
+    ("vis_save", C.c_int, [C.POINTER(VisT), C.c_char_p, C.c_int]),
+    ("vis_draw", C.c_int, [C.POINTER(VisT), C.POINTER(SimT)]),
     ("sim_find", C.POINTER(C.c_float), [C.POINTER(SimT), C.c_int]),
     ("sim_init", C.c_int, [C.POINTER(SimT), C.c_uint]),
     ("sim_free", C.c_int, [C.POINTER(SimT)]),

@@ -22,6 +22,8 @@ typedef struct {
 } sf_node;
 
 /* --- reference octree (tree_ref.hip) --- */
+/* the reference's FP64 Kahan mean radius into avg[0] (r1: N doubles of scratch) */
+int sfd_mean_radius(const float* pos, unsigned int N, double* r1, double* avg, hipStream_t s);
 /* r1: N doubles of scratch */
 int sfd_tree_bounds_ref(const float* pos, unsigned int N, const double jitter[3], double* r1, float* bounds,
                         hipStream_t s);

@@ -36,14 +36,12 @@ __device__ __forceinline__ void kahan_step(double& avg, double& com, double r) {
 	avg = t;
 }
 
-// Mean radius (the sequential FP64 Kahan sum of src/simulation.c:245-277,
-// one wavefront; the radii arrive through scalar loads, 16 ahead of the
-// chain) and the jittered cube bounds (src/simulation.c:279-301).
-// jitter[i] is 2 * (INV_PCG32_MAX * s[i]) - 1 from pcg4d of the step number.
+// Mean radius: the sequential FP64 Kahan sum of src/simulation.c:245-277
+// (and src/visualization.c:276-305) over the radii, in one wavefront; the
+// radii arrive through scalar loads, 16 ahead of the chain.
 constexpr unsigned kKahanAhead = 16;
 
-__global__ __launch_bounds__(64) void k_tree_bounds_ref(const double* __restrict__ r1, unsigned N, double j0, double j1,
-                                                        double j2, float* __restrict__ bounds) {
+__device__ double kahan_mean(const double* __restrict__ r1, unsigned N) {
 	double avg = 0.0, com = 0.0;
 	unsigned i = 0;
 	if (N >= kKahanAhead) {
@@ -64,13 +62,27 @@ __global__ __launch_bounds__(64) void k_tree_bounds_ref(const double* __restrict
 		i += kKahanAhead;
 	}
 	for (; i < N; ++i) kahan_step(avg, com, r1[i]);
+	return avg * (1.0 / (double)N);
+}
+
+// The jittered cube bounds (src/simulation.c:279-301) around the mean
+// radius; jitter[i] is 2 * (INV_PCG32_MAX * s[i]) - 1 from pcg4d of the step
+// number.
+__global__ __launch_bounds__(64) void k_tree_bounds_ref(const double* __restrict__ r1, unsigned N, double j0, double j1,
+                                                        double j2, float* __restrict__ bounds) {
+	const double avg = kahan_mean(r1, N);
 	if (threadIdx.x != 0) return;
-	avg *= 1.0 / (double)N;
 	const double jit[3] = {j0, j1, j2};
 	for (int a = 0; a < 3; ++a) {
 		bounds[a] = (float)(-2.000 * avg + 0.5 * jit[a] * avg);
 		bounds[3 + a] = (float)(2.000 * avg + 0.5 * jit[a] * avg);
 	}
+}
+
+__global__ __launch_bounds__(64) void k_mean_radius(const double* __restrict__ r1, unsigned N,
+                                                    double* __restrict__ avg) {
+	const double a = kahan_mean(r1, N);
+	if (threadIdx.x == 0) avg[0] = a;
 }
 
 // Serial insertion (src/tree.c:120-296).  meta[0] = num_nodes,
@@ -383,6 +395,13 @@ extern "C" int sfd_tree_bounds_ref(const float* pos, unsigned N, const double ji
 	hipLaunchKernelGGL(k_radius_f64, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pos, N, r1);
 	hipLaunchKernelGGL(k_tree_bounds_ref, dim3(1), dim3(64), 0, s, r1, N, jitter[0], jitter[1], jitter[2], bounds);
 	return sf_hip_check(hipGetLastError(), "sfd_tree_bounds_ref", "launch");
+}
+
+extern "C" int sfd_mean_radius(const float* pos, unsigned N, double* r1, double* avg, hipStream_t s) {
+	if (N == 0) return 0;
+	hipLaunchKernelGGL(k_radius_f64, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pos, N, r1);
+	hipLaunchKernelGGL(k_mean_radius, dim3(1), dim3(64), 0, s, r1, N, avg);
+	return sf_hip_check(hipGetLastError(), "sfd_mean_radius", "launch");
 }
 
 extern "C" int sfd_tree_build_ref(const float* pos, const float* mas, unsigned N, const float* bounds,

@@ -12,7 +12,13 @@
 /* 2^-32 rounded down to a binary32 value, so u * SF_INV_U32 < 1 (src/rng.h:9) */
 #define SF_INV_U32 2.3283062977608182109179324470460414886474609375e-10
 
-static inline void sf_pcg4d(uint32_t s[4]) {
+#ifdef __HIPCC__
+#define SF_RNG_FN static inline __host__ __device__
+#else
+#define SF_RNG_FN static inline
+#endif
+
+SF_RNG_FN void sf_pcg4d(uint32_t s[4]) {
 	uint32_t x = s[0] * 1664525u + 1013904223u;
 	uint32_t y = s[1] * 1664525u + 1013904223u;
 	uint32_t z = s[2] * 1664525u + 1013904223u;

@@ -15,6 +15,10 @@ ref).  The fixtures are data: inputs and the reference's outputs.
   f4_tree_n4096.npz       reference octree at N = 4096: node arrays, acc and
                           pot after sim_solv (step 0) and after one step
   f5_pcg4d.npz            pcg4d known answers from the reference's rng.h
+  f6_vis_n4096.npz        vis_draw of the F2 uniform ball after its initial
+                          sim_solv at step 7, 192 x 128 pixels: the atomic and
+                          render buffers, matMVP, the PPM file vis_save
+                          writes and the 8-bit pixels of its PNG conversion
 
     python tests/golden/make_golden.py
 """
@@ -183,8 +187,26 @@ def f5():
     np.savez_compressed(GOLD / "f5_pcg4d.npz", inputs=ins, outputs=outs)
 
 
+def f6():
+    N = 4096
+    r = O.RefSim("pp_e1e-2_dt1e-3", N)
+    r.mem[:] = config1_ic(N)
+    r.sim.step_number = 0
+    assert r.solve() == 0
+    r.sim.step_number = 7
+    # no PFM: in this PNG-format build the reference's PFM path writes past
+    # its 3-byte-per-pixel binary buffer
+    with tempfile.TemporaryDirectory() as d:
+        ppm, png = Path(d) / "f.ppm", Path(d) / "f.png"
+        atomic, render, mvp, png_pixels = r.vis_frame(192, 128, files=((ppm, 1), (png, 2)))
+        ppm_bytes = np.frombuffer(ppm.read_bytes(), np.uint8)
+    np.savez_compressed(GOLD / "f6_vis_n4096.npz", mem=r.mem.copy(), step=7, sizex=192, sizey=128, atomic=atomic,
+                        render=render, mvp=mvp, ppm=ppm_bytes, png_pixels=png_pixels, eps=1e-2, dt=1e-3)
+    r.close()
+
+
 if __name__ == "__main__":
     O.build(with_reference=True)
-    for fn in (f1, f2, f3, f4, f5):
+    for fn in (f1, f2, f3, f4, f5, f6):
         fn()
         print("wrote", fn.__name__)

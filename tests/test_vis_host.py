@@ -1,0 +1,118 @@
+"""The visualisation boundary without a GPU: vis_t's layout against the
+reference header, and vis_save (src/visualization_io.c:176-330) on a render
+buffer filled here, against the reference build's own vis_save: PFM and PPM
+files byte for byte, and the PNG's pixels against the reference's 8-bit PNG
+conversion (the reference writes them with stb, which is not available, so
+the PNG container is checked on its own)."""
+import ctypes as C
+import struct
+import subprocess
+import tempfile
+import zlib
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from starflood_amd import _lib as L
+from starflood_amd.vis import Visualization
+
+REF_SRC = Path("/root/reference/src")
+
+VIS_PROBE = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "visualization.h"
+int main(void) {
+	printf("%zu %zu %zu %zu %zu %zu %zu %d %d %d\n", sizeof(vis_t), offsetof(vis_t, sizex), offsetof(vis_t, sizey),
+	       offsetof(vis_t, mem), offsetof(vis_t, atomic_buffer), offsetof(vis_t, render_buffer),
+	       offsetof(vis_t, binary_buffer), (int)IMAGE_FILETYPE_PFM, (int)IMAGE_FILETYPE_PPM, (int)IMAGE_FILETYPE_PNG);
+	return 0;
+}
+"""
+
+
+@pytest.mark.skipif(not REF_SRC.exists(), reason="reference sources not present")
+def test_vis_t_layout_matches_reference_header():
+    with tempfile.TemporaryDirectory() as d:
+        src = Path(d) / "probe.c"
+        src.write_text(VIS_PROBE)
+        exe = Path(d) / "probe"
+        subprocess.run(["gcc", "-std=c99", f"-I{REF_SRC}", str(src), "-o", str(exe)], check=True, capture_output=True)
+        ref = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    T = L.VisT
+    ours = [C.sizeof(T), T.sizex.offset, T.sizey.offset, T.mem.offset, T.atomic_buffer.offset,
+            T.render_buffer.offset, T.binary_buffer.offset, L.IMAGE_FILETYPE_PFM, L.IMAGE_FILETYPE_PPM,
+            L.IMAGE_FILETYPE_PNG]
+    assert ours == ref
+
+
+def render_pattern(w, h, seed):
+    """Values across the transfer curves' branches, the clamps and exact 0/1."""
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(-0.2, 1.2, size=(h, w, 4)).astype(np.float32)
+    r[0, :, :3] = np.linspace(0.0, 0.02, w, dtype=np.float32)[:, None]    # both linear segments
+    r[1, :4, :3] = [[0.0] * 3, [1.0] * 3, [0.0031308] * 3, [0.018053968510807] * 3]
+    r[..., 3] = 1.0
+    return r
+
+
+def decode_png(data: bytes) -> tuple[int, int, np.ndarray]:
+    """Minimal PNG reader for 8-bit RGB with filter 0 rows; checks every CRC."""
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, w, h = 8, b"", None, None
+    while pos < len(data):
+        n, kind = struct.unpack(">I4s", data[pos:pos + 8])
+        body = data[pos + 8:pos + 8 + n]
+        (crc,) = struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])
+        assert crc == zlib.crc32(kind + body)
+        if kind == b"IHDR":
+            w, h, depth, ctype, comp, filt, inter = struct.unpack(">IIBBBBB", body)
+            assert (depth, ctype, comp, filt, inter) == (8, 2, 0, 0, 0)
+        elif kind == b"IDAT":
+            idat += body
+        pos += 12 + n
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, 3 * w + 1)   # zlib checks Adler-32
+    assert (raw[:, 0] == 0).all()
+    return w, h, raw[:, 1:].copy()
+
+
+@pytest.fixture
+def ref_vis_save(oracle):
+    if not oracle.ref_available("pp_e1e-2_dt1e-3"):
+        pytest.skip("reference build not present")
+    return oracle.ref_vis_save
+
+
+def pfm_bytes(render) -> bytes:
+    """src/visualization_io.c:27-51 with :207-216: header, then the RGB
+    floats in buffer row order."""
+    h, w = render.shape[:2]
+    return f"PF\n{w} {h}\n-1.0\n".encode() + np.ascontiguousarray(render[..., :3], np.float32).tobytes()
+
+
+@pytest.mark.parametrize("w,h", [(37, 23), (300, 250)])
+def test_vis_save_matches_reference_files(ref_vis_save, tmp_path, w, h):
+    """PPM bytes and PNG pixels against the reference build's vis_save; the
+    PFM against its two-line statement (the reference's PFM path overruns its
+    buffer in a PNG-format build).  (300, 250) spans several 64 KiB stored
+    deflate blocks."""
+    render = render_pattern(w, h, w)
+    with Visualization(w, h) as v:
+        v.render[:] = render
+        v.save(tmp_path / "ours.ppm", L.IMAGE_FILETYPE_PPM)
+        v.save(tmp_path / "ours.pfm", L.IMAGE_FILETYPE_PFM)
+        v.save(tmp_path / "ours.png", L.IMAGE_FILETYPE_PNG)
+    ref_vis_save(render, L.IMAGE_FILETYPE_PPM, tmp_path / "ref.ppm")
+    assert (tmp_path / "ours.ppm").read_bytes() == (tmp_path / "ref.ppm").read_bytes()
+    assert (tmp_path / "ours.pfm").read_bytes() == pfm_bytes(render)
+    pw, ph, pixels = decode_png((tmp_path / "ours.png").read_bytes())
+    assert (pw, ph) == (w, h)
+    assert np.array_equal(pixels.ravel(), ref_vis_save(render, L.IMAGE_FILETYPE_PNG))
+
+
+def test_vis_save_reports_unwritable_path_and_bad_type(tmp_path):
+    with Visualization(4, 4) as v:
+        with pytest.raises(RuntimeError):
+            v.save(tmp_path / "missing_dir" / "x.ppm", L.IMAGE_FILETYPE_PPM)
+        assert L.load().vis_save(C.byref(v.vis), str(tmp_path / "x").encode(), 7) == -1
