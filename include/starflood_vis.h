@@ -45,7 +45,7 @@ int vis_init(vis_t* SF_RESTRICT vis_ptr, const unsigned int sizex, const unsigne
 /* reference src/visualization.h:33 (src/visualization.c:192-230) */
 int vis_free(vis_t* SF_RESTRICT vis_ptr);
 
-/* reference src/visualization.h:36 (src/visualization_io.c:176-330):
+/* reference src/visualization.h:36 (src/visualization_io.c:166-330):
    writes the render buffer as PFM (RGB floats), PPM (8-bit, Rec. 709
    transfer) or PNG (8-bit, sRGB transfer; stored deflate blocks).  Rows
    are flipped for PPM and PNG as in the reference.  Fails when the file

@@ -275,7 +275,7 @@ def ref_vis_save(render, filetype: int, path=None):
     the build has no PNG writer), and the file it writes when path is
     given.  Not PFM: in this (PNG-format) build the binary buffer holds
     3 bytes per pixel and the PFM path writes 12 (src/visualization.c:76-87,
-    src/visualization_io.c:207-216), past the end of the allocation."""
+    src/visualization_io.c:195-204), past the end of the allocation."""
     if filetype == 0:
         raise ValueError("the reference's PFM path overflows its buffer in this build")
     lib = C.CDLL(str(REF_DIR / "libref_pp_e1e-2_dt1e-3.so"))

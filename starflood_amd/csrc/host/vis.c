@@ -322,7 +322,7 @@ static int write_ppm(const unsigned char* rgb, size_t w, size_t h, const char* f
 	return rc;
 }
 
-/* reference src/visualization_io.c:176-330 */
+/* reference src/visualization_io.c:166-330 */
 int vis_save(const vis_t* SF_RESTRICT vis_ptr, const char* SF_RESTRICT filename, image_filetype_t filetype) {
 	if (!vis_ptr || !vis_ptr->mem || !filename) return STARFLOOD_FAILURE;
 	if (filetype != IMAGE_FILETYPE_PFM && filetype != IMAGE_FILETYPE_PPM && filetype != IMAGE_FILETYPE_PNG) {

@@ -1,5 +1,5 @@
 """The visualisation boundary without a GPU: vis_t's layout against the
-reference header, and vis_save (src/visualization_io.c:176-330) on a render
+reference header, and vis_save (src/visualization_io.c:166-330) on a render
 buffer filled here, against the reference build's own vis_save: PFM and PPM
 files byte for byte, and the PNG's pixels against the reference's 8-bit PNG
 conversion (the reference writes them with stb, which is not available, so
@@ -85,7 +85,7 @@ def ref_vis_save(oracle):
 
 
 def pfm_bytes(render) -> bytes:
-    """src/visualization_io.c:27-51 with :207-216: header, then the RGB
+    """src/visualization_io.c:27-51 with :195-204: header, then the RGB
     floats in buffer row order."""
     h, w = render.shape[:2]
     return f"PF\n{w} {h}\n-1.0\n".encode() + np.ascontiguousarray(render[..., :3], np.float32).tobytes()
