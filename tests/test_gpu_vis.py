@@ -62,3 +62,29 @@ def test_large_frame_is_deterministic_and_tone_mapped_from_its_counts():
     expect = np.tanh(scale * a1[..., :3].astype(np.float64)).astype(np.float32)
     assert ulp_diff(r1[..., :3], expect).max() <= 1
     assert (r1[..., 3] == 1.0).all()
+
+
+def test_degenerate_frames_and_positions():
+    """A 0 x 0 and a 1 x 1 frame; a particle at 3e38 (the mean radius, and so
+    the zoom, follows it); and NaN or infinite coordinates, which make the
+    reference's mean radius, hence its whole camera matrix, NaN, so that no
+    sample lands (a NaN pixel coordinate converts to INT_MIN on its host)."""
+    N = 256
+    with Simulation(N, enable_tree=0, logging=0, epsilon=1e-2, dt=1e-3) as s:
+        s.ic_uniform_ball(seed=9, radius=1.0, total_mass=1.0, v_hubble=0.0)
+        s.solve()
+        s.pos[2] = (3e38, 3e38, 3e38)
+        with Visualization(0, 0) as v:
+            v.draw(s)
+        with Visualization(1, 1) as v:
+            v.draw(s)
+            a = v.atomic.copy()
+            assert 0 <= a[0, 0, 3] <= 128 * N and (a[..., :3] <= 100 * a[..., 3:4]).all()
+        with Visualization(64, 48) as v:
+            v.draw(s)
+            assert 0 < v.atomic[..., 3].sum() <= 128 * (N - 1)
+            for bad in ((np.nan, 0.0, 0.0), (np.inf, -np.inf, 0.0)):
+                s.pos[0] = bad
+                v.draw(s)
+                assert v.atomic.sum() == 0
+                assert (v.render[..., :3] == 0.0).all() and (v.render[..., 3] == 1.0).all()
