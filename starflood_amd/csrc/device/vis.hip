@@ -91,10 +91,11 @@ __global__ __launch_bounds__(kBlock) void k_vis_splat(const float* __restrict__ 
 		const unsigned long long same = __ballot(in_frame && pixel == p) & active;
 		if ((int)lane == leader) {
 			const int cnt = __popcll(same);
-			atomicAdd(&buf[4u * p + 0u], cnt * col[0]);
-			atomicAdd(&buf[4u * p + 1u], cnt * col[1]);
-			atomicAdd(&buf[4u * p + 2u], cnt * col[2]);
-			atomicAdd(&buf[4u * p + 3u], cnt);
+			int* px = buf + 4u * (size_t)p;
+			atomicAdd(&px[0], cnt * col[0]);
+			atomicAdd(&px[1], cnt * col[1]);
+			atomicAdd(&px[2], cnt * col[2]);
+			atomicAdd(&px[3], cnt);
 		}
 		active &= ~same;
 	}
