@@ -18,7 +18,11 @@
 // per interaction, several i per lane for instruction-level parallelism, j
 // fetched with scalar loads, partial sums per tile of SF_TJ interactions
 // folded into a Kahan-compensated running sum, and the j range split into
-// slices so the grid fills all 256 CUs.  Slice s is processed by blocks with
+// slices so the grid fills all 256 CUs.  When every particle has the same
+// mass m (k_pack_bodies checks the bits of each mass on the device), the
+// whole tiles of real particles drop the per-pair mass product: they sum
+// 1/s^3 (r_j - r_i) and 1/s with 12 VALU operations plus the rsqrt, and the
+// tile partial is scaled by m before the fold.  Slice s is processed by blocks with
 // blockIdx % slices == s, which the dispatcher places on XCD (s mod 8), so
 // each XCD's L2 holds only its own slice of the body array.  A second kernel
 // combines the slice partials per i in fixed order in FP64, which keeps the
@@ -96,9 +100,54 @@ __global__ __launch_bounds__(kBlock) void k_pp_exact(const float* __restrict__ p
 	}
 }
 
+// One tile of SF_TJ interactions, j from jt.  nb holds the tile's first
+// kPrefetch bodies on entry and the next tile's first ones on exit (the
+// group after the slice's last is its own first again: in bounds; the
+// compiler then issues the four loads as separate dwordx4 into fresh
+// registers, early in the group -- measured faster than one dwordx16 into
+// the registers being consumed).  UNIFORM: every mass of the tile equals m,
+// and the sums leave it out (sum_j (r_j - r_i)/s^3 and sum_j 1/s).
+template <int IPT, bool WITH_POT, bool UNIFORM>
+__device__ __forceinline__ void pp_tile(const float4* __restrict__ body, unsigned jt, unsigned j1,
+                                        float4 (&nb)[kPrefetch], const float (&xi)[IPT], const float (&yi)[IPT],
+                                        const float (&zi)[IPT], float eps2, float (&tx)[IPT], float (&ty)[IPT],
+                                        float (&tz)[IPT], float (&tp)[IPT]) {
+	for (unsigned j = jt; j < jt + SF_TJ; j += kPrefetch) {
+		float4 cb[kPrefetch];
+#pragma unroll
+		for (int u = 0; u < kPrefetch; ++u) cb[u] = nb[u];
+		const unsigned jn = (j + kPrefetch < j1) ? j + kPrefetch : j;
+#pragma unroll
+		for (int u = 0; u < kPrefetch; ++u) nb[u] = body[jn + u];
+#pragma unroll
+		for (int u = 0; u < kPrefetch; ++u) {
+			const float4 bj = cb[u];
+#pragma unroll
+			for (int k = 0; k < IPT; ++k) {
+				const float dx = bj.x - xi[k];
+				const float dy = bj.y - yi[k];
+				const float dz = bj.z - zi[k];
+				float r2 = __builtin_fmaf(dx, dx, eps2);
+				r2 = __builtin_fmaf(dy, dy, r2);
+				r2 = __builtin_fmaf(dz, dz, r2);
+				const float inv = __builtin_amdgcn_rsqf(r2);
+				const float mi = UNIFORM ? inv : bj.w * inv;
+				const float inv2 = inv * inv;
+				const float w = mi * inv2;
+				tx[k] = __builtin_fmaf(w, dx, tx[k]);
+				ty[k] = __builtin_fmaf(w, dy, ty[k]);
+				tz[k] = __builtin_fmaf(w, dz, tz[k]);
+				if (WITH_POT) tp[k] += mi;
+			}
+		}
+	}
+}
+
 // Accumulates, for IPT particles per lane, sum_j m_j (r_j - r_i) / s^3 and
 // sum_j m_j / s over one j slice.  part holds, per slice, two planes of
-// n_pad float4: the running sum and its Kahan compensation.
+// n_pad float4: the running sum and its Kahan compensation.  The word after
+// the n_pad bodies holds n_uni: the bodies [0, n_uni) are whole tiles of
+// particles of one mass, body[0].w (0 when the masses differ).
 template <int IPT, bool WITH_POT>
 __global__ __launch_bounds__(kBlock) void k_pp_fast(const float4* __restrict__ body, unsigned n_pad,
                                                     unsigned slices, unsigned slice_len, float eps2,
@@ -123,49 +172,36 @@ __global__ __launch_bounds__(kBlock) void k_pp_fast(const float4* __restrict__ b
 
 	// The j stream keeps one group of kPrefetch bodies in flight: the scalar
 	// loads of the next group are issued before the current group's
-	// arithmetic, so their latency overlaps it.
+	// arithmetic, so their latency overlaps it.  The equal-mass tiles come
+	// first in a loop of their own: one loop with a branch per tile made the
+	// compiler carry the j stream in VGPRs (measured 8 % slower).
 	const unsigned j0 = s * slice_len;
 	const unsigned j1 = min(j0 + slice_len, n_pad);
+	const unsigned n_uni = *reinterpret_cast<const unsigned*>(body + n_pad);
+	const unsigned ju = min(j1, max(j0, n_uni));
+	const float mu = body[0].w;
 	float4 nb[kPrefetch];
 #pragma unroll
 	for (int u = 0; u < kPrefetch; ++u) nb[u] = body[j0 + u];
-	for (unsigned jt = j0; jt < j1; jt += SF_TJ) {
+	unsigned jt = j0;
+	for (; jt < ju; jt += SF_TJ) {
 		float tx[IPT], ty[IPT], tz[IPT], tp[IPT];
 #pragma unroll
 		for (int k = 0; k < IPT; ++k) tx[k] = ty[k] = tz[k] = tp[k] = 0.0f;
-		for (unsigned j = jt; j < jt + SF_TJ; j += kPrefetch) {
-			float4 cb[kPrefetch];
+		pp_tile<IPT, WITH_POT, true>(body, jt, j1, nb, xi, yi, zi, eps2, tx, ty, tz, tp);
 #pragma unroll
-			for (int u = 0; u < kPrefetch; ++u) cb[u] = nb[u];
-			// the group after the slice's last is its own first again (in bounds;
-			// the compiler then issues the four loads as separate dwordx4 into
-			// fresh registers, early in the group -- measured faster than one
-			// dwordx16 into the registers being consumed)
-			const unsigned jn = (j + kPrefetch < j1) ? j + kPrefetch : j;
-#pragma unroll
-			for (int u = 0; u < kPrefetch; ++u) nb[u] = body[jn + u];
-#pragma unroll
-			for (int u = 0; u < kPrefetch; ++u) {
-				const float4 bj = cb[u];
-#pragma unroll
-				for (int k = 0; k < IPT; ++k) {
-					const float dx = bj.x - xi[k];
-					const float dy = bj.y - yi[k];
-					const float dz = bj.z - zi[k];
-					float r2 = __builtin_fmaf(dx, dx, eps2);
-					r2 = __builtin_fmaf(dy, dy, r2);
-					r2 = __builtin_fmaf(dz, dz, r2);
-					const float inv = __builtin_amdgcn_rsqf(r2);
-					const float mi = bj.w * inv;
-					const float inv2 = inv * inv;
-					const float w = mi * inv2;
-					tx[k] = __builtin_fmaf(w, dx, tx[k]);
-					ty[k] = __builtin_fmaf(w, dy, ty[k]);
-					tz[k] = __builtin_fmaf(w, dz, tz[k]);
-					if (WITH_POT) tp[k] += mi;
-				}
-			}
+		for (int k = 0; k < IPT; ++k) {
+			kahan_add(Sx[k], Cx[k], mu * tx[k]);
+			kahan_add(Sy[k], Cy[k], mu * ty[k]);
+			kahan_add(Sz[k], Cz[k], mu * tz[k]);
+			if (WITH_POT) kahan_add(Sp[k], Cp[k], mu * tp[k]);
 		}
+	}
+	for (; jt < j1; jt += SF_TJ) {
+		float tx[IPT], ty[IPT], tz[IPT], tp[IPT];
+#pragma unroll
+		for (int k = 0; k < IPT; ++k) tx[k] = ty[k] = tz[k] = tp[k] = 0.0f;
+		pp_tile<IPT, WITH_POT, false>(body, jt, j1, nb, xi, yi, zi, eps2, tx, ty, tz, tp);
 #pragma unroll
 		for (int k = 0; k < IPT; ++k) {
 			kahan_add(Sx[k], Cx[k], tx[k]);
@@ -210,12 +246,39 @@ __global__ __launch_bounds__(kBlock) void k_pp_combine(const float4* __restrict_
 	if (pot) pot[i] = (float)(-G * sp);
 }
 
+// Packs (x, y, z, m) per body, padding with zeros to n_pad, and finds
+// whether every mass has the bits of mas[0].  st[0] receives n_uni, the
+// whole SF_TJ tiles of [0, N) when they do, else 0 (padding bodies, mass 0,
+// are never in a uniform tile).  st[1] collects the blocks' differing-mass
+// flags and st[2] counts finished blocks; the last block to finish writes
+// st[0] and returns both to zero (atomicInc wraps at gridDim.x - 1), so
+// they start at zero on the next call without a clear.  The grid is capped
+// at kPackBlocks and strides over the bodies: the atomics on st[2] serialise
+// (one per block of 1024 measured 26 us at N = 262144).
+constexpr unsigned kPackBlocks = 128;
 __global__ __launch_bounds__(kBlock) void k_pack_bodies(const float* __restrict__ pos, const float* __restrict__ mas,
-                                                        unsigned N, unsigned n_pad, float4* __restrict__ body) {
-	const unsigned i = blockIdx.x * kBlock + threadIdx.x;
-	if (i >= n_pad) return;
-	body[i] = i < N ? make_float4(pos[3u * i], pos[3u * i + 1u], pos[3u * i + 2u], mas[i])
-	                : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                                                        unsigned N, unsigned n_pad, float4* __restrict__ body,
+                                                        unsigned* __restrict__ st) {
+	const unsigned m0 = N ? __float_as_uint(mas[0]) : 0u;
+	int d = 0;
+	for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < n_pad; i += gridDim.x * kBlock) {
+		if (i < N) {
+			const float m = mas[i];
+			body[i] = make_float4(pos[3u * i], pos[3u * i + 1u], pos[3u * i + 2u], m);
+			d |= __float_as_uint(m) != m0;
+		} else {
+			body[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		}
+	}
+	d = __syncthreads_or(d);
+	if (threadIdx.x == 0) {
+		if (d) atomicOr(&st[1], 1u);
+		__threadfence();
+		if (atomicInc(&st[2], gridDim.x - 1u) == gridDim.x - 1u) {
+			const unsigned any = atomicExch(&st[1], 0u);
+			st[0] = any ? 0u : N / SF_TJ * SF_TJ;
+		}
+	}
 }
 
 // FP64 direct sum: one block per listed particle, lanes stride over j, then
@@ -272,9 +335,10 @@ int env_uint(const char* name, int fallback) {
 
 extern "C" int sfd_pack_bodies(const float* pos, const float* mas, unsigned N, unsigned n_pad, float* body,
                                hipStream_t s) {
-	const unsigned grid = (n_pad + kBlock - 1) / kBlock;
+	const unsigned grid = min((n_pad + kBlock - 1) / kBlock, kPackBlocks);
+	unsigned* st = reinterpret_cast<unsigned*>(body + 4u * (size_t)n_pad);
 	hipLaunchKernelGGL(k_pack_bodies, dim3(grid), dim3(kBlock), 0, s, pos, mas, N, n_pad,
-	                   reinterpret_cast<float4*>(body));
+	                   reinterpret_cast<float4*>(body), st);
 	return sf_hip_check(hipGetLastError(), "sfd_pack_bodies", "k_pack_bodies launch");
 }
 

@@ -51,9 +51,16 @@ static int scratch_alloc(sf_pp_scratch* w, unsigned int N, const char* func) {
 	w->n_pad = ((N ? N : 1u) + 1023u) / 1024u * 1024u;
 	w->slices = sf_pp_slice_count(w->n_pad);
 	const size_t plane = (size_t)w->n_pad * 4u * sizeof(float);
-	if (hipMalloc((void**)&w->body, plane) != hipSuccess || hipMalloc((void**)&w->nu, plane) != hipSuccess ||
+	if (hipMalloc((void**)&w->body, SF_BODY_BYTES(w->n_pad)) != hipSuccess ||
+	    hipMalloc((void**)&w->nu, plane) != hipSuccess ||
 	    hipMalloc((void**)&w->part, plane * 2u * w->slices) != hipSuccess) {
 		sf_error(func, "device allocation for N = %u failed", N);
+		scratch_free(w);
+		return -1;
+	}
+	/* the packed-body state words start at zero (set-up only, then synchronised) */
+	if (hipMemset(w->body, 0, SF_BODY_BYTES(w->n_pad)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+		sf_error(func, "clearing the body buffer for N = %u failed", N);
 		scratch_free(w);
 		return -1;
 	}

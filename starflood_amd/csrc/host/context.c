@@ -65,11 +65,12 @@ sf_ctx* sf_ctx_create(void* host_mem, unsigned int N, unsigned int dof) {
 	const size_t nm = (size_t)N * dof;
 	if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) goto fail;
 	if (hipMalloc((void**)&c->d_mem, (nm ? nm : 1u) * sizeof(float)) != hipSuccess) goto fail;
-	if (hipMalloc((void**)&c->d_body, (size_t)c->n_pad * 4u * sizeof(float)) != hipSuccess) goto fail;
+	if (hipMalloc((void**)&c->d_body, SF_BODY_BYTES(c->n_pad)) != hipSuccess) goto fail;
 	if (hipMalloc((void**)&c->d_x64, ((size_t)N * 3u + 1u) * sizeof(double)) != hipSuccess) goto fail;
 	if (hipMalloc((void**)&c->d_red, (size_t)sfd_reduce_scratch_doubles(N) * sizeof(double)) != hipSuccess) goto fail;
 	if (hipHostMalloc((void**)&c->h_red, SF_RED_VALUES * sizeof(double), hipHostMallocDefault) != hipSuccess) goto fail;
 	if (hipMemsetAsync(c->d_mem, 0, (nm ? nm : 1u) * sizeof(float), c->stream) != hipSuccess) goto fail;
+	if (hipMemsetAsync(c->d_body, 0, SF_BODY_BYTES(c->n_pad), c->stream) != hipSuccess) goto fail;
 	if (hipMemsetAsync(c->d_x64, 0, ((size_t)N * 3u + 1u) * sizeof(double), c->stream) != hipSuccess) goto fail;
 	if (hipStreamSynchronize(c->stream) != hipSuccess) goto fail;
 	g_ctx[slot] = c;

@@ -39,7 +39,7 @@ typedef struct sf_ctx {
 	unsigned int dof;        /* reals per particle in sim.mem */
 	unsigned int n_pad;      /* N rounded up to the PP j tile */
 	float* d_mem;            /* device copy of sim.mem, same layout */
-	float* d_body;           /* 4 * n_pad floats: x, y, z, m (pad: m = 0) */
+	float* d_body;           /* SF_BODY_BYTES(n_pad): x, y, z, m (pad: m = 0), then SF_BODY_STATE words */
 	double* d_x64;           /* 3N FP64 positions for the FAST drift */
 	float* d_part;           /* PP partial sums, 8 * slices * n_pad floats */
 	unsigned int part_slices;
@@ -90,6 +90,11 @@ void sf_prof_end(int token, hipStream_t s);
 
 /* ---------- device launchers (csrc/device) ---------- */
 #define SF_PP_TILE 256u      /* j padding granule of d_body */
+/* packed bodies: n_pad float4, then SF_BODY_STATE words that sfd_pack_bodies
+   reads and writes (the leading bodies that form whole tiles of equal mass,
+   and two block-count words that must be zero when the buffer is made) */
+#define SF_BODY_STATE 64u
+#define SF_BODY_BYTES(n_pad) ((size_t)(n_pad) * 16u + SF_BODY_STATE * 4u)
 #define SF_RED_VALUES 16u    /* doubles per reduction partial */
 
 /* body[i] = (pos_i, m_i) for i < N, zero for N <= i < n_pad */

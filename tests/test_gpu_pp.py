@@ -110,6 +110,30 @@ def test_fast_pp_close_to_fp64_small(oracle, N):
     L.set_params()
 
 
+@pytest.mark.parametrize("N", [127, 128, 1000, 4097])
+@pytest.mark.parametrize("masses", ["equal", "one_differs", "zero_first"])
+def test_fast_pp_equal_mass_tiles_close_to_fp64(oracle, N, masses):
+    """Equal masses take the kernel's mass-free tiles (whole 128-body tiles of
+    real particles; the tile holding the padding keeps the general sum); one
+    differing mass anywhere sends every tile through the general sum."""
+    rng = np.random.default_rng(1000 + N)
+    pos = rng.normal(size=(N, 3)).astype(np.float32)
+    mas = np.full(N, 1.0 / 3.0 / N, dtype=np.float32)
+    if masses == "one_differs":
+        mas[N // 2] *= np.float32(1.5)
+    elif masses == "zero_first":
+        mas[0] = 0.0
+    with exact_sim(N, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL) as s:
+        s.pos[:] = pos
+        s.mas[:] = mas
+        s.solve_gravity_pp()
+        a64, p64 = oracle.pp_f64(pos, mas, 1e-2, acc_phys=1)
+        scale = np.linalg.norm(a64, axis=1).max() + 1e-30
+        assert np.abs(s.acc - a64).max() <= 2e-6 * scale
+        assert np.allclose(s.pot, p64, rtol=2e-6)
+    L.set_params()
+
+
 def test_f64_direct_sum_kernel_matches_cpu_oracle(oracle):
     N = 8192
     with exact_sim(N, epsilon=2e-3, acc_mode=L.SF_ACC_PHYSICAL) as s:
