@@ -132,6 +132,13 @@ def cpu_baseline(budget_s: float = 12.0) -> dict:
         out["reference_tree_solve"] = {"N": REFTREE_N, "ms": t * 1e3,
                                        "mem_sha256": hashlib.sha256(r.mem.tobytes()).hexdigest()}
         r.close()
+    if O.ref_program_available(RUN_PROGRAM):
+        # the reference program's own run of the sf_run workload
+        import tempfile
+        with tempfile.TemporaryDirectory() as d:
+            secs = O.run_ref_program(RUN_PROGRAM, Path(d), threads)
+            out["reference_run"] = {"N": RUN_N, "steps": RUN_STEPS, "s": secs,
+                                    "snapshot_sha256": snapshot_hashes(Path(d))}
     if O.ref_available("pp_e1e-2_dt1e-3"):
         # the reference's own vis_draw of the same frame
         r = O.RefSim("pp_e1e-2_dt1e-3", VIS_N)
@@ -230,6 +237,45 @@ def bench_vis(reps: int = 5) -> dict:
            "samples_in_frame": int(v.atomic[..., 3].sum())}
     v.close()
     s.close()
+    return out
+
+
+RUN_N = 16384
+RUN_STEPS = 10
+RUN_PROGRAM = "pp_n16384_s10"   # the reference program built for this run (src/main.c)
+
+
+def snapshot_hashes(d: Path) -> list[str]:
+    return [hashlib.sha256(f.read_bytes()).hexdigest() for f in sorted((d / "out" / "sim").glob("*.raw"))]
+
+
+def bench_run() -> dict:
+    """The reference program's whole run loop (sf_run: initcond_generate,
+    sim_solv, then 10 steps with a snapshot before each and after the last)
+    at N = 16384, eps 0.01, dt 1e-3, PP gravity, written to a temporary
+    directory: the exact policy in mirror mode (host sim.mem authoritative
+    after every call, as the reference) and the fast policy resident."""
+    import tempfile
+    from starflood_amd.run import run
+    out = {"workload": f"sf_run: the reference program's loop, initcond_generate N={RUN_N}, {RUN_STEPS} KDK steps of "
+                       "PP gravity (eps 0.01, dt 1e-3), a raw snapshot every step, statistics.csv and "
+                       "timings_sim_step.csv", "N": RUN_N, "steps": RUN_STEPS}
+    for key, params in (("exact_mirror", dict(precision=L.SF_PRECISION_EXACT)),
+                        ("fast_resident", dict(precision=L.SF_PRECISION_FAST, mem_mode=L.SF_MEM_RESIDENT))):
+        with tempfile.TemporaryDirectory() as d:
+            d = Path(d)
+            (d / "out" / "sim").mkdir(parents=True)
+            hip_sync()
+            t0 = time.perf_counter()
+            rc = run(RUN_N, RUN_STEPS, out_dir=str(d / "out"), enable_tree=0, epsilon=1e-2, dt=1e-3, logging=1,
+                     **params)
+            hip_sync()
+            out[f"{key}_s"] = time.perf_counter() - t0
+            if rc != 0:
+                raise RuntimeError(f"sf_run returned {rc}")
+            if key == "exact_mirror":
+                out["snapshot_sha256"] = snapshot_hashes(d)
+    L.set_params()
     return out
 
 
@@ -473,7 +519,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the BH, adjoint and SPH measurements")
     ap.add_argument("--only", choices=["bh", "adjoint", "sph", "sphgrid", "copy", "bh1m", "stress", "reftree",
-                                       "vis"],
+                                       "vis", "run"],
                     help="run one secondary workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -485,7 +531,7 @@ def main():
               "sph": lambda: bench_sph(steps=args.steps),
               "sphgrid": lambda: bench_sph(steps=args.steps, search=L.SF_SPH_SEARCH_GRID), "copy": bench_copy,
               "bh1m": bench_bh1m,
-              "stress": bench_stress, "reftree": bench_reftree, "vis": bench_vis}[args.only]
+              "stress": bench_stress, "reftree": bench_reftree, "vis": bench_vis, "run": bench_run}[args.only]
         print(json.dumps(fn()))
         return
 
@@ -539,7 +585,7 @@ def main():
     if not args.no_extra:
         out["extra"] = {"bh_4m": bench_bh(), "adjoint_16k": bench_adjoint(), "sph_65k": bench_sph(),
                         "sph_65k_grid": bench_sph(search=L.SF_SPH_SEARCH_GRID), "reftree_65k": bench_reftree(),
-                        "vis_65k": bench_vis()}
+                        "vis_65k": bench_vis(), "run_16k": bench_run()}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
         ref_tree = out["cpu_baseline"].get("reference_tree_solve")
@@ -550,6 +596,11 @@ def main():
         ref_vis = out["cpu_baseline"].get("reference_vis_draw")
         if ref_vis and "extra" in out:
             out["extra"]["vis_65k"]["reference_cpu_draw_ms"] = ref_vis["ms"]
+        ref_run = out["cpu_baseline"].get("reference_run")
+        if ref_run and "extra" in out:
+            rr = out["extra"]["run_16k"]
+            rr["reference_program_s"] = ref_run["s"]
+            rr["snapshots_identical_to_reference"] = rr.pop("snapshot_sha256") == ref_run.pop("snapshot_sha256")
     print(json.dumps(out))
 
 

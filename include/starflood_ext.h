@@ -13,6 +13,7 @@
 #define STARFLOOD_EXT_H
 
 #include "starflood.h"
+#include "starflood_vis.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -225,6 +226,40 @@ void sf_tape_free(sf_tape* tape);
 int sf_force_dev(unsigned int N, const float* x, const float* m, float* acc, float* pot, void* stream);
 /* Force vector-Jacobian product: g = (d acc / d x)^T mu, device arrays. */
 int sf_force_vjp_dev(unsigned int N, const float* x, const float* m, const float* mu, float* g, void* stream);
+
+/* --- the reference program's run loop (src/main.c:354-611) ---
+   The reference sets these with config.h macros; the physics parameters
+   come from sf_set_params, and every path is under its out_dir
+   (OUTPUT_DIR). */
+typedef struct {
+	unsigned int num_bodies;        /* NUM_BODIES, config.h:161 */
+	unsigned int num_timesteps;     /* NUM_TIMESTEPS, config.h:22: steps 0 .. num_timesteps */
+	unsigned int output_interval;   /* OUTPUT_INTERVAL, config.h:78; 0 = undefined (every step) */
+	int enable_sim;                 /* ENABLE_SIM, config.h:26 */
+	int enable_vis;                 /* ENABLE_VIS, config.h:29 */
+	const char* sim_filename;       /* SIM_FILENAME, config.h:85 ("sim/step_%04u"; ".raw" is
+	                                   appended); NULL = undefined: no snapshots */
+	const char* vis_filename;       /* VIS_FILENAME, config.h:88 ("vis/step_%04u"; the file
+	                                   type's extension is appended); NULL = no frames */
+	image_filetype_t vis_file_type; /* VIS_FILE_FORMAT, config.h:108 */
+	unsigned int vis_sizex, vis_sizey; /* visualization_dimensions, src/main.c:81-99 */
+	const char* init_cond_file;     /* INIT_COND_FILE, config.h:82; NULL = initcond_generate */
+	int verbose;                    /* print the per-step progress lines of src/main.c:424-425 */
+} sf_run_config_t;
+
+/* Replaces the body of the reference's main() (src/main.c:348-611): set up
+   the visualisation and the simulation, apply the initial conditions (read
+   init_cond_file; else initcond_generate and one sim_solv), then for each
+   step: stop if <out_dir>/stop exists (and remove it), set step_number,
+   save the snapshot (before the step; or read it when enable_sim is 0),
+   vis_draw, sim_step (all but the last step), un-premultiply and vis_save.
+   With enable_sim 0 and an output interval, steps advance by the interval.
+   A file name pattern holds exactly one %u conversion, which receives
+   step / output_interval (or the step).  Returns 0 when the run ends or
+   stops, 1 when it ends early because a replayed snapshot could not be read
+   or a file name did not fit (the reference's program then still exits with
+   success), -1 when the set-up fails. */
+int sf_run(const sf_run_config_t* cfg);
 
 #ifdef __cplusplus
 }

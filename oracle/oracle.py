@@ -46,13 +46,14 @@ class OrcSim(C.Structure):
 
 def build(with_reference: bool | None = None) -> None:
     """Compile liboracle.so and, when the reference sources are present,
-    the reference variants into _ref/."""
+    the reference variants and the reference program's run configurations
+    into _ref/."""
     subprocess.run(["make", "-s", "-C", str(ORACLE_DIR), "-B", "all"], check=True)
     if with_reference is None:
         with_reference = REFERENCE_SRC.is_dir()
     if with_reference:
-        subprocess.run(["make", "-s", "-C", str(ORACLE_DIR), "ref", f"REF_SRC={REFERENCE_SRC}"], check=True,
-                       stdout=subprocess.DEVNULL)
+        subprocess.run(["make", "-s", "-C", str(ORACLE_DIR), "ref", "ref_main", f"REF_SRC={REFERENCE_SRC}"],
+                       check=True, stdout=subprocess.DEVNULL)
 
 
 _lib = None
@@ -297,6 +298,23 @@ def ref_vis_save(render, filetype: int, path=None):
 
 def ref_available(variant: str) -> bool:
     return (REF_DIR / f"libref_{variant}.so").exists()
+
+
+def run_ref_program(name: str, workdir: Path, threads: int) -> float:
+    """Run the reference program build _ref/ref_main_<name> (src/main.c with
+    that run configuration) in workdir, where it writes ./out/; create
+    out/sim and out/vis first.  Returns the wall time in seconds."""
+    exe = REF_DIR / f"ref_main_{name}"
+    (Path(workdir) / "out" / "sim").mkdir(parents=True, exist_ok=True)
+    (Path(workdir) / "out" / "vis").mkdir(parents=True, exist_ok=True)
+    env = {"OMP_NUM_THREADS": str(threads), "PATH": os.environ.get("PATH", "/usr/bin:/bin")}
+    t0 = time.perf_counter()
+    subprocess.run([str(exe)], cwd=str(workdir), env=env, check=True, stdout=subprocess.DEVNULL)
+    return time.perf_counter() - t0
+
+
+def ref_program_available(name: str) -> bool:
+    return (REF_DIR / f"ref_main_{name}").exists()
 
 
 @contextmanager

@@ -13,6 +13,9 @@
  *   SFREF_TREE                     keep ENABLE_TREE (else direct summation)
  *   SFREF_SPH                      define ENABLE_SPH
  *   SFREF_TREE_NODES_MAX           override TREE_NODES_MAX
+ *   SFREF_NUM_BODIES, SFREF_NUM_TIMESTEPS, SFREF_OUTPUT_INTERVAL
+ *                                  the run of src/main.c (the program
+ *                                  builds in _ref/ref_main_*)
  */
 #include "config.h"
 
@@ -43,4 +46,19 @@
 #ifdef SFREF_TREE_NODES_MAX
 #undef TREE_NODES_MAX
 #define TREE_NODES_MAX SFREF_TREE_NODES_MAX
+#endif
+
+#ifdef SFREF_NUM_BODIES
+#undef NUM_BODIES
+#define NUM_BODIES SFREF_NUM_BODIES
+#endif
+
+#ifdef SFREF_NUM_TIMESTEPS
+#undef NUM_TIMESTEPS
+#define NUM_TIMESTEPS SFREF_NUM_TIMESTEPS
+#endif
+
+#ifdef SFREF_OUTPUT_INTERVAL
+#undef OUTPUT_INTERVAL
+#define OUTPUT_INTERVAL SFREF_OUTPUT_INTERVAL
 #endif

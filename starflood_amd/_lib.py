@@ -68,6 +68,15 @@ class VisT(C.Structure):
 IMAGE_FILETYPE_PFM, IMAGE_FILETYPE_PPM, IMAGE_FILETYPE_PNG = 0, 1, 2
 
 
+class RunConfig(C.Structure):
+    """sf_run_config_t: the config.h settings of the reference program's run
+    loop (src/main.c)."""
+    _fields_ = [("num_bodies", C.c_uint), ("num_timesteps", C.c_uint), ("output_interval", C.c_uint),
+                ("enable_sim", C.c_int), ("enable_vis", C.c_int), ("sim_filename", C.c_char_p),
+                ("vis_filename", C.c_char_p), ("vis_file_type", C.c_int), ("vis_sizex", C.c_uint),
+                ("vis_sizey", C.c_uint), ("init_cond_file", C.c_char_p), ("verbose", C.c_int)]
+
+
 class Stats(C.Structure):
     """sf_stats_t."""
     _fields_ = [("T", C.c_double), ("V", C.c_double), ("E", C.c_double), ("W", C.c_double), ("U", C.c_double),
@@ -119,6 +128,7 @@ _SIG = [
     ("sf_tape_free", None, [C.c_void_p]),
     ("sf_force_dev", C.c_int, [C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("sf_force_vjp_dev", C.c_int, [C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("sf_run", C.c_int, [C.POINTER(RunConfig)]),
 ]
 
 _lib = None

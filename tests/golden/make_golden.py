@@ -19,6 +19,11 @@ ref).  The fixtures are data: inputs and the reference's outputs.
                           sim_solv at step 7, 192 x 128 pixels: the atomic and
                           render buffers, matMVP, the PPM file vis_save
                           writes and the 8-bit pixels of its PNG conversion
+  f7_run_n256.npz         the reference program's run loop (src/main.c) at
+                          N = 256, 10 steps, eps 1e-2, dt 1e-3, every step and
+                          every 4th step: the snapshot files it writes (names,
+                          SHA-256, the last one whole), statistics.csv, and
+                          the step column of timings_sim_step.csv
 
     python tests/golden/make_golden.py
 """
@@ -205,8 +210,26 @@ def f6():
     r.close()
 
 
+def f7():
+    out = {}
+    for key, exe in (("every", "ref_main_pp_n256_s10"), ("oi4", "ref_main_pp_n256_s10_oi4")):
+        with tempfile.TemporaryDirectory() as d:
+            (Path(d) / "out" / "sim").mkdir(parents=True)
+            env = {"OMP_NUM_THREADS": "4", "PATH": "/usr/bin:/bin"}
+            subprocess.run([str(REPO / "oracle" / "_ref" / exe)], cwd=d, env=env, check=True,
+                           stdout=subprocess.DEVNULL)
+            files = sorted((Path(d) / "out" / "sim").glob("*.raw"))
+            out[f"{key}_names"] = np.array([f.name for f in files])
+            out[f"{key}_sha256"] = np.array([hashlib.sha256(f.read_bytes()).hexdigest() for f in files])
+            out[f"{key}_last"] = np.frombuffer(files[-1].read_bytes(), np.float32).copy()
+            out[f"{key}_statistics"] = np.array((Path(d) / "out" / "statistics.csv").read_text())
+            tl = (Path(d) / "out" / "timings_sim_step.csv").read_text().splitlines()
+            out[f"{key}_timing_steps"] = np.array([int(l.split(",")[0]) for l in tl[1:]])
+    np.savez_compressed(GOLD / "f7_run_n256.npz", N=256, num_timesteps=10, eps=1e-2, dt=1e-3, **out)
+
+
 if __name__ == "__main__":
     O.build(with_reference=True)
-    for fn in (f1, f2, f3, f4, f5, f6):
+    for fn in (f1, f2, f3, f4, f5, f6, f7):
         fn()
         print("wrote", fn.__name__)

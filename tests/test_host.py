@@ -36,3 +36,22 @@ def test_lbvh_export_without_a_tree_fails_cleanly():
     lib = L.load()
     t = L.TreeT()
     assert lib.sf_lbvh_export(C.byref(t), None, None, None, None, None) == -1
+
+
+def test_sf_run_rejects_bad_configurations_before_any_device_work(tmp_path):
+    """sf_run checks its configuration first (no simulation is made)."""
+    import ctypes as C
+    from starflood_amd import _lib as L
+    lib = L.load()
+    assert lib.sf_run(None) == L.STARFLOOD_FAILURE
+    L.set_params(out_dir=str(tmp_path))
+    for pattern in (b"sim/step_%s", b"sim/%u_%u", b"sim/step", b"sim/%.3u", b"sim/%n"):
+        cfg = L.RunConfig(num_bodies=16, num_timesteps=1, enable_sim=1, sim_filename=pattern)
+        assert lib.sf_run(C.byref(cfg)) == L.STARFLOOD_FAILURE, pattern
+        assert "%u" in L.last_error()
+    cfg = L.RunConfig(num_bodies=16, num_timesteps=1, enable_sim=1, enable_vis=1, vis_sizex=0, vis_sizey=8)
+    assert lib.sf_run(C.byref(cfg)) == L.STARFLOOD_FAILURE
+    cfg = L.RunConfig(num_bodies=16, num_timesteps=1, enable_sim=1, vis_file_type=7)
+    assert lib.sf_run(C.byref(cfg)) == L.STARFLOOD_FAILURE
+    assert list(tmp_path.iterdir()) == []
+    L.set_params()
