@@ -408,6 +408,28 @@ def bench_stress() -> dict:
     return out
 
 
+def bench_pp_mirror(steps: int = 5) -> dict:
+    """The headline PP step in mirror mode: host sim.mem is authoritative
+    after every call, as the reference's boundary promises, so each sim_step
+    moves its blocks over PCIe (pageable host memory, the caller's own
+    buffer).  The PCIe-inclusive rate; not the bench value."""
+    sim = Simulation(N_PP, enable_tree=0, logging=0, verbose=0, epsilon=EPS, dt=DT,
+                     precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL, mem_mode=L.SF_MEM_MIRROR)
+    sim.ic_plummer(seed=SEED, first=0, count=N_PP)
+    sim.solve()
+    sim.step()
+    hip_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    hip_sync()
+    elapsed = time.perf_counter() - t0
+    sim.close()
+    return {"workload": f"PP gravity KDK step as the headline, N={N_PP}, mirror mode (host sim.mem authoritative "
+                        "after each sim_step)", "steps": steps, "ms_per_step": elapsed / steps * 1e3,
+            "Ginteractions_per_s": float(N_PP) * N_PP * steps / elapsed / 1e9}
+
+
 def bench_copy(mib: int = 1024) -> dict:
     """A plain device copy of known size (torch), the calibration run for the
     FETCH_SIZE / WRITE_SIZE counters."""
@@ -519,7 +541,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the BH, adjoint and SPH measurements")
     ap.add_argument("--only", choices=["bh", "adjoint", "sph", "sphgrid", "copy", "bh1m", "stress", "reftree",
-                                       "vis", "run"],
+                                       "vis", "run", "mirror"],
                     help="run one secondary workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -531,7 +553,8 @@ def main():
               "sph": lambda: bench_sph(steps=args.steps),
               "sphgrid": lambda: bench_sph(steps=args.steps, search=L.SF_SPH_SEARCH_GRID), "copy": bench_copy,
               "bh1m": bench_bh1m,
-              "stress": bench_stress, "reftree": bench_reftree, "vis": bench_vis, "run": bench_run}[args.only]
+              "stress": bench_stress, "reftree": bench_reftree, "vis": bench_vis, "run": bench_run,
+              "mirror": bench_pp_mirror}[args.only]
         print(json.dumps(fn()))
         return
 
@@ -585,7 +608,7 @@ def main():
     if not args.no_extra:
         out["extra"] = {"bh_4m": bench_bh(), "adjoint_16k": bench_adjoint(), "sph_65k": bench_sph(),
                         "sph_65k_grid": bench_sph(search=L.SF_SPH_SEARCH_GRID), "reftree_65k": bench_reftree(),
-                        "vis_65k": bench_vis(), "run_16k": bench_run()}
+                        "vis_65k": bench_vis(), "run_16k": bench_run(), "pp_262k_mirror": bench_pp_mirror()}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
         ref_tree = out["cpu_baseline"].get("reference_tree_solve")
