@@ -169,6 +169,8 @@ int sf_lbvh_walk(sf_lbvh* t, unsigned int N, const float* pos, const float* mas,
 	                     p->acc_mode == SF_ACC_PHYSICAL, acc, pot, t->counters, t->gstack, s);
 }
 
+unsigned int sf_lbvh_size(const sf_lbvh* t) { return t ? t->N : 0u; }
+
 int sf_lbvh_export_impl(sf_lbvh* t, hipStream_t s, unsigned long long* keys, unsigned int* order,
                         unsigned long long counts[4], float* node_cm, int* node_lr_range) {
 	if (!t || t->N == 0) {

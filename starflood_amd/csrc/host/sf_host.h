@@ -37,6 +37,8 @@ int sf_lbvh_build(sf_lbvh** st, unsigned int N, const float* pos, const float* m
 int sf_lbvh_walk(sf_lbvh* st, unsigned int N, const float* pos, const float* mas, const sf_params_t* p, float* acc,
                  float* pot, hipStream_t s);
 void sf_lbvh_destroy(sf_lbvh* st);
+/* particles of the LBVH last built (0: none, or no build yet) */
+unsigned int sf_lbvh_size(const sf_lbvh* st);
 /* synchronises s, then copies out what is asked for (NULL: skip) */
 int sf_lbvh_export_impl(sf_lbvh* st, hipStream_t s, unsigned long long* keys, unsigned int* order,
                         unsigned long long counts[4], float* node_cm, int* node_lr_range);

@@ -143,6 +143,7 @@ static int download_solver_outputs(sf_ctx* c) {
 int sf_solve_all(sf_ctx* c, tree_t* tree, unsigned int step_number, float sph_pred_dt) {
 	const sf_params_t* p = sf_params();
 	int rc = 0;
+	if (c->N == 0) return 0;   /* no particles: no forces, no densities */
 	if (p->enable_sph && sf_sph_prepare(c, sph_pred_dt)) return -1;
 	if (p->enable_tree) {
 		if (sf_solve_gravity_tree(c, tree, step_number)) rc = -1;
@@ -166,7 +167,7 @@ int sim_solv(sim_t* SF_RESTRICT sim_ptr) {
 	if (sf_solve_all(c, &sim_ptr->tree, sim_ptr->step_number, 0.0f)) rc = -1;
 	if (download_solver_outputs(c)) rc = -1;
 	if (sf_finish(c, "sim_solv")) rc = -1;
-	if (sf_params()->enable_tree && sf_tree_check(&sim_ptr->tree, c->stream)) rc = -1;
+	if (sf_params()->enable_tree && c->N && sf_tree_check(&sim_ptr->tree, c->stream)) rc = -1;
 	return rc ? STARFLOOD_FAILURE : STARFLOOD_SUCCESS;
 }
 

@@ -32,7 +32,7 @@ static int ensure_tmp(sf_ctx* c) {
 
 int sf_sph_prepare(sf_ctx* c, float pred_dt) {
 	const sf_params_t* p = sf_params();
-	if (!p->enable_sph || p->sph_model != SF_SPH_CUBIC_IDEAL) return 0;
+	if (!p->enable_sph || p->sph_model != SF_SPH_CUBIC_IDEAL || c->N == 0) return 0;
 	if (ensure_tmp(c) || sf_ctx_ensure_u(c)) return -1;
 	const unsigned int N = c->N;
 	return sfd_sph_cubic_pack(sf_blk(c->d_mem, N, SF_E_POS), sf_blk(c->d_mem, N, SF_E_VEL),
@@ -44,6 +44,7 @@ int sf_sph_prepare(sf_ctx* c, float pred_dt) {
 int sf_solve_sph(sf_ctx* c) {
 	const sf_params_t* p = sf_params();
 	const unsigned int N = c->N;
+	if (N == 0) return 0;
 	if (ensure_tmp(c)) return -1;
 	float* rho = sf_blk(c->d_mem, N, SF_E_RHO);
 	float* prs = sf_blk(c->d_mem, N, SF_E_PRS);

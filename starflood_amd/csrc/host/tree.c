@@ -149,7 +149,8 @@ static int is_device_pointer(const void* p) {
    reference's errors (src/tree.c:180-189) after a synchronisation. */
 static int build_ref(sf_tree_dev* d, tree_t* t, unsigned int N, const float* pos, const float* mas, hipStream_t s) {
 	const sf_params_t* p = sf_params();
-	const size_t need = sfd_tree_par_bytes(N, d->max_nodes);   /* 0: serial build only */
+	/* 0: serial build only (also for no particles: the root alone) */
+	const size_t need = N ? sfd_tree_par_bytes(N, d->max_nodes) : 0u;
 	d->order_n = 0u;
 	if (need && d->par_bytes < need) {
 		if (d->par.scratch) {
@@ -271,6 +272,10 @@ int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number) {
 	const sf_params_t* p = sf_params();
 	const unsigned int N = c->N;
 	hipStream_t s = c->stream;
+	if (N == 0) {   /* no particles: an empty tree, no forces */
+		tree->num_nodes = 0u;
+		return 0;
+	}
 	const float* pos = sf_blk(c->d_mem, N, SF_E_POS);
 	const float* mas = sf_blk(c->d_mem, N, SF_E_MAS);
 	float* acc = sf_blk(c->d_mem, N, SF_E_ACC);
@@ -313,6 +318,7 @@ int sf_lbvh_export(const tree_t* tree_ptr, unsigned long long* keys, unsigned in
    tree (synchronises s). */
 static int lbvh_check(sf_tree_dev* d, hipStream_t s) {
 	unsigned long long counts[4] = {0, 0, 0, 0};
+	if (d->lbvh && sf_lbvh_size(d->lbvh) == 0u) return 0;   /* no particles: nothing was walked */
 	if (!d->lbvh || sf_lbvh_export_impl(d->lbvh, s, NULL, NULL, counts, NULL, NULL)) return -1;
 	if (counts[2]) {
 		sf_error("solve_gravity_part_tree", "%llu wavefronts met a malformed tree", counts[2]);
