@@ -38,8 +38,9 @@ typedef struct {
 } vis_t;
 
 /* reference src/visualization.h:30 (src/visualization.c:37-190): allocates
-   and zeroes the buffers (4 MiB aligned); the device copies are made by the
-   first vis_draw. */
+   and zeroes the buffers (4 MiB aligned; page-locked when a device is
+   present, so vis_draw copies back at full rate); the device copies are made
+   by the first vis_draw. */
 int vis_init(vis_t* SF_RESTRICT vis_ptr, const unsigned int sizex, const unsigned int sizey);
 
 /* reference src/visualization.h:33 (src/visualization.c:192-230) */

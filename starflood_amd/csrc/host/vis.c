@@ -46,6 +46,14 @@ typedef struct {
 
 static sf_vis_dev* g_vis[SF_VIS_MAX];
 
+/* vis_init's page-locked buffers (the device copies back into them at full
+   PCIe rate): the aligned address handed out, and the allocation to free */
+typedef struct {
+	void* aligned;
+	void* base;
+} sf_vis_pinned;
+static sf_vis_pinned g_pinned[SF_VIS_MAX];
+
 static void vis_dev_destroy(sf_vis_dev* d) {
 	if (!d) return;
 	for (int k = 0; k < SF_VIS_MAX; ++k)
@@ -95,10 +103,23 @@ int vis_init(vis_t* SF_RESTRICT vis_ptr, const unsigned int sizex, const unsigne
 	/* atomic (i32 RGBA), render (f32 RGBA), binary (room for f32 RGB) */
 	const size_t mem_size = 16u * px + 16u * px + 12u * px;
 	void* mem = NULL;
-	if (posix_memalign(&mem, (size_t)SF_VIS_ALIGN, mem_size ? mem_size : 1u) != 0 || !mem) {
-		fprintf(stderr, "%s error: mem is NULL after posix_memalign(&mem, %zu, %zu)\n", "vis_init()",
-		        (size_t)SF_VIS_ALIGN, mem_size);
-		return STARFLOOD_FAILURE;
+	/* page-locked when a device is present and a slot is free, else
+	   ordinary host memory (as the reference's aligned_alloc) */
+	int slot = -1;
+	for (int k = 0; k < SF_VIS_MAX && slot < 0; ++k)
+		if (!g_pinned[k].base) slot = k;
+	void* base = NULL;
+	if (slot >= 0 && hipHostMalloc(&base, mem_size + SF_VIS_ALIGN, hipHostMallocDefault) == hipSuccess && base) {
+		mem = (void*)(((uintptr_t)base + (SF_VIS_ALIGN - 1u)) & ~(uintptr_t)(SF_VIS_ALIGN - 1u));
+		g_pinned[slot].aligned = mem;
+		g_pinned[slot].base = base;
+	} else {
+		(void)hipGetLastError();
+		if (posix_memalign(&mem, (size_t)SF_VIS_ALIGN, mem_size ? mem_size : 1u) != 0 || !mem) {
+			fprintf(stderr, "%s error: mem is NULL after posix_memalign(&mem, %zu, %zu)\n", "vis_init()",
+			        (size_t)SF_VIS_ALIGN, mem_size);
+			return STARFLOOD_FAILURE;
+		}
 	}
 	memset(mem, 0, mem_size ? mem_size : 1u);
 	vis_ptr->sizex = sizex;
@@ -120,7 +141,14 @@ int vis_free(vis_t* SF_RESTRICT vis_ptr) {
 				vis_dev_destroy(g_vis[k]);
 			}
 	}
-	free(vis_ptr->mem);
+	int pinned = 0;
+	for (int k = 0; k < SF_VIS_MAX && vis_ptr->mem; ++k)
+		if (g_pinned[k].aligned == vis_ptr->mem) {
+			hipHostFree(g_pinned[k].base);
+			g_pinned[k].aligned = g_pinned[k].base = NULL;
+			pinned = 1;
+		}
+	if (!pinned) free(vis_ptr->mem);
 	vis_ptr->sizex = 0u;
 	vis_ptr->sizey = 0u;
 	vis_ptr->mem = NULL;
