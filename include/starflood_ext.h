@@ -227,7 +227,7 @@ int sf_force_dev(unsigned int N, const float* x, const float* m, float* acc, flo
 /* Force vector-Jacobian product: g = (d acc / d x)^T mu, device arrays. */
 int sf_force_vjp_dev(unsigned int N, const float* x, const float* m, const float* mu, float* g, void* stream);
 
-/* --- the reference program's run loop (src/main.c:354-611) ---
+/* --- the reference program's run loop (src/main.c:348-611) ---
    The reference sets these with config.h macros; the physics parameters
    come from sf_set_params, and every path is under its out_dir
    (OUTPUT_DIR). */

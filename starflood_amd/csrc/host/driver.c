@@ -1,5 +1,5 @@
 /*
- * driver.c -- the reference program's run loop (src/main.c:354-611) as a
+ * driver.c -- the reference program's run loop (src/main.c:348-611) as a
  * library call, over this library's sim_* and vis_* entry points.
  *
  * Order of one step, as src/main.c:423-591:
@@ -60,7 +60,7 @@ static const char* vis_ext(image_filetype_t t) {
 	}
 }
 
-/* src/main.c:553-576: divide RGB by alpha (when non-zero), alpha = 1 */
+/* src/main.c:558-582: divide RGB by alpha (when non-zero), alpha = 1 */
 static void unpremultiply(vis_t* v) {
 	const size_t n = (size_t)v->sizex * v->sizey;
 	float* px = v->render_buffer;
@@ -141,7 +141,7 @@ int sf_run(const sf_run_config_t* cfg) {
 		}
 	}
 
-	/* src/main.c:414-419 */
+	/* src/main.c:414-421 */
 	const unsigned int step_inc = (!enable_sim && oi) ? oi : 1u;
 	for (unsigned int step = 0u; step <= cfg->num_timesteps; step += step_inc) {
 		if (cfg->verbose) {
@@ -149,7 +149,7 @@ int sf_run(const sf_run_config_t* cfg) {
 			       100.0 * (0u < cfg->num_timesteps ? (double)step / (double)cfg->num_timesteps : 1.0));
 			printf("t = %.06f\n", p.dt * (double)step);
 		}
-		/* src/main.c:426-445: the stop file ends the run and is removed */
+		/* src/main.c:427-446: the stop file ends the run and is removed */
 		FILE* stopfile = fopen(stop_name, "r");
 		if (stopfile) {
 			if (cfg->verbose) printf("\nStopfile found! Stopping run...\n");
@@ -167,7 +167,7 @@ int sf_run(const sf_run_config_t* cfg) {
 			rc = 1;
 			break;
 		}
-		/* src/main.c:495-535: snapshot before the step, or replay */
+		/* src/main.c:487-528: snapshot before the step, or replay */
 		if (output_step && sim_io) {
 			if (enable_sim) {
 				if (STARFLOOD_SUCCESS != sim_save_raw(&sim, sim_name))
@@ -178,13 +178,13 @@ int sf_run(const sf_run_config_t* cfg) {
 				break;
 			}
 		}
-		/* src/main.c:537-546 */
+		/* src/main.c:530-539 */
 		if (output_step && enable_vis && STARFLOOD_SUCCESS != vis_draw(&vis, &sim))
 			fprintf(stderr, "error: vis_draw(&vis, &sim) failed.\n");
-		/* src/main.c:548-557 */
+		/* src/main.c:541-550 */
 		if (step < cfg->num_timesteps && enable_sim && STARFLOOD_SUCCESS != sim_step(&sim))
 			fprintf(stderr, "error: sim_step(&sim) failed.\n");
-		/* src/main.c:559-585 */
+		/* src/main.c:552-588 */
 		if (output_step && enable_vis && vis_io) {
 			unpremultiply(&vis);
 			if (STARFLOOD_SUCCESS != vis_save(&vis, vis_name, cfg->vis_file_type))

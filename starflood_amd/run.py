@@ -1,4 +1,4 @@
-"""The reference program's run loop (src/main.c:354-611) for Python callers:
+"""The reference program's run loop (src/main.c:348-611) for Python callers:
 the config.h settings of a run as arguments, the physics parameters as
 keywords (the defaults are config.h's), one call to sf_run."""
 from __future__ import annotations

@@ -1,4 +1,4 @@
-"""The reference program's run loop (src/main.c:354-611) through sf_run,
+"""The reference program's run loop (src/main.c:348-611) through sf_run,
 against the files the reference program itself writes (golden fixture F7:
 its snapshots, statistics.csv and the step column of timings_sim_step.csv at
 N = 256, 10 steps, every step and every 4th step)."""
@@ -65,7 +65,7 @@ def test_stop_file_ends_the_run_and_is_removed(tmp_path):
 
 
 def test_replay_reads_snapshots_and_draws_frames(tmp_path):
-    """ENABLE_SIM off (src/main.c:414-419, 512-534): steps advance by the
+    """ENABLE_SIM off (src/main.c:414-421, 503-514): steps advance by the
     output interval and snapshot step / interval is read, drawn and saved;
     a missing snapshot ends the run early (return 1)."""
     d = out_dir(tmp_path)
