@@ -277,11 +277,29 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
 				if (nl >= 64 || (drain && (nl > 0 || npid > 0))) {
 					int q = 0;
 					while (q < nl || (drain && npid > 0)) {
-						while (q < nl && npid < 64) {
-							const int2 lf = leaves[q];
-							if ((int)lane < lf.y) pid[npid + (int)lane] = lf.x + (int)lane;
-							npid += lf.y;
-							++q;
+						if (q < nl && npid < 64) {
+							// up to 64 leaves at once, one per lane: an inclusive scan of
+							// their counts places each leaf's particle ids after the
+							// previous ones; the leaves that fit in pid[] (a prefix, the
+							// scan being nondecreasing, and at least the first: counts
+							// are at most 64 and npid < 64) are taken
+							const int li = q + (int)lane;
+							const int2 lf = li < nl ? leaves[li] : make_int2(0, 0);
+							int incl = lf.y;
+#pragma unroll
+							for (int o = 1; o < 64; o <<= 1) {
+								const int v = __shfl_up(incl, (unsigned)o);
+								if ((int)lane >= o) incl += v;
+							}
+							const bool fits = li < nl && npid + incl <= kList;
+							const int k = __popcll(__ballot(fits));
+							const int total = __shfl(incl, k - 1);
+							if (fits) {
+								const int at = npid + incl - lf.y;
+								for (int j = 0; j < lf.y; ++j) pid[at + j] = lf.x + j;
+							}
+							npid += total;
+							q += k;
 						}
 						wave_fence();
 						if (npid >= 64 || (drain && q == nl && npid > 0)) {
