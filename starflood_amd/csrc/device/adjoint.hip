@@ -11,10 +11,13 @@
 // w = m_k nu_j - m_j nu_k is formed from two rounded products, so the self
 // pair (j = k, where K = I/eps^3 is largest) cancels exactly.
 //
-// Per pair: 30 VALU operations and one v_rsq_f32 (the force takes 13), with
+// Per pair: 26 VALU operations and one v_rsq_f32 (the force takes 13), with
 // the same layout as the fast PP kernel: one lane per k, the j stream
 // wavefront-uniform (scalar loads), j split into slices over the grid with
-// Kahan-folded partial sums per tile, and a fixed-order FP64 combine.
+// Kahan-folded partial sums per tile, and a fixed-order FP64 combine.  The
+// whole tiles of equal-mass particles that sfd_pack_bodies finds form
+// w = m (nu_j - nu_k) instead (23 VALU operations; still exactly zero for
+// j == k), scaling each tile partial by m.
 #include <hip/hip_runtime.h>
 
 #include "../sf_internal.h"
@@ -34,10 +37,60 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
 	s = t;
 }
 
-// body = (x, y, z, m) and nu = (nu, 0) for n_pad entries (m = nu = 0 past N)
+// One tile of kTile pairs, j from jt; nbb/nbn hold the tile's first
+// kPrefetch j records on entry and the next tile's on exit.  UNIFORM: every
+// mass of the tile and of k equals m, and the sums leave it out.
+template <bool UNIFORM>
+__device__ __forceinline__ void vjp_tile(const float4* __restrict__ body, const float4* __restrict__ nu, unsigned jt,
+                                         unsigned j1, float4 (&nbb)[kPrefetch], float4 (&nbn)[kPrefetch],
+                                         const float4& bk, const float4& nk, float eps2, float& tx, float& ty,
+                                         float& tz) {
+	for (unsigned j = jt; j < jt + kTile; j += kPrefetch) {
+		float4 cbb[kPrefetch], cbn[kPrefetch];
+#pragma unroll
+		for (int u = 0; u < kPrefetch; ++u) {
+			cbb[u] = nbb[u];
+			cbn[u] = nbn[u];
+		}
+		const unsigned jn = (j + kPrefetch < j1) ? j + kPrefetch : j;   // in bounds
+#pragma unroll
+		for (int u = 0; u < kPrefetch; ++u) {
+			nbb[u] = body[jn + u];
+			nbn[u] = nu[jn + u];
+		}
+#pragma unroll
+		for (int u = 0; u < kPrefetch; ++u) {
+			const float4 bj = cbb[u];
+			const float4 nj = cbn[u];
+			const float dx = bk.x - bj.x, dy = bk.y - bj.y, dz = bk.z - bj.z;
+			float r2 = __builtin_fmaf(dx, dx, eps2);
+			r2 = __builtin_fmaf(dy, dy, r2);
+			r2 = __builtin_fmaf(dz, dz, r2);
+			const float inv = __builtin_amdgcn_rsqf(r2);
+			const float inv2 = inv * inv;
+			const float inv3 = inv2 * inv;
+			const float i5_3 = inv3 * (3.0f * inv2);
+			// exactly zero when j == k: a difference of equal values, or of two
+			// equal rounded products
+			const float wx = UNIFORM ? nj.x - nk.x : bk.w * nj.x - bj.w * nk.x;
+			const float wy = UNIFORM ? nj.y - nk.y : bk.w * nj.y - bj.w * nk.y;
+			const float wz = UNIFORM ? nj.z - nk.z : bk.w * nj.z - bj.w * nk.z;
+			const float dw = __builtin_fmaf(dz, wz, __builtin_fmaf(dy, wy, dx * wx));
+			const float q = i5_3 * dw;
+			tx = __builtin_fmaf(-q, dx, __builtin_fmaf(inv3, wx, tx));
+			ty = __builtin_fmaf(-q, dy, __builtin_fmaf(inv3, wy, ty));
+			tz = __builtin_fmaf(-q, dz, __builtin_fmaf(inv3, wz, tz));
+		}
+	}
+}
+
+// body = (x, y, z, m) and nu = (nu, 0) for n_pad entries (m = nu = 0 past
+// N); the word after the n_pad bodies holds n_uni, the leading bodies that
+// form whole tiles of one mass, body[0].w (sfd_pack_bodies)
 __global__ __launch_bounds__(kBlock) void k_vjp(const float4* __restrict__ body, const float4* __restrict__ nu,
                                                 unsigned n_pad, unsigned slices, unsigned slice_len, float eps2,
                                                 float4* __restrict__ part) {
+	static_assert(kTile == 128u, "n_uni counts whole tiles of 128 bodies");
 	const unsigned b = blockIdx.x;
 	const unsigned sl = b % slices;
 	const unsigned k = (b / slices) * kBlock + threadIdx.x;
@@ -47,50 +100,32 @@ __global__ __launch_bounds__(kBlock) void k_vjp(const float4* __restrict__ body,
 	// one group of kPrefetch j records in flight, as in the fast PP kernel
 	const unsigned j0 = sl * slice_len;
 	const unsigned j1 = min(j0 + slice_len, n_pad);
+	const unsigned n_uni = *reinterpret_cast<const unsigned*>(body + n_pad);
+	const unsigned ju = min(j1, max(j0, n_uni));
+	const float mu = body[0].w;
 	float4 nbb[kPrefetch], nbn[kPrefetch];
 #pragma unroll
 	for (int u = 0; u < kPrefetch; ++u) {
 		nbb[u] = body[j0 + u];
 		nbn[u] = nu[j0 + u];
 	}
-	for (unsigned jt = j0; jt < j1; jt += kTile) {
-		float tx = 0.0f, ty = 0.0f, tz = 0.0f;
-		for (unsigned j = jt; j < jt + kTile; j += kPrefetch) {
-			float4 cbb[kPrefetch], cbn[kPrefetch];
-#pragma unroll
-			for (int u = 0; u < kPrefetch; ++u) {
-				cbb[u] = nbb[u];
-				cbn[u] = nbn[u];
-			}
-			const unsigned jn = (j + kPrefetch < j1) ? j + kPrefetch : j;   // in bounds
-#pragma unroll
-			for (int u = 0; u < kPrefetch; ++u) {
-				nbb[u] = body[jn + u];
-				nbn[u] = nu[jn + u];
-			}
-#pragma unroll
-			for (int u = 0; u < kPrefetch; ++u) {
-				const float4 bj = cbb[u];
-				const float4 nj = cbn[u];
-				const float dx = bk.x - bj.x, dy = bk.y - bj.y, dz = bk.z - bj.z;
-				float r2 = __builtin_fmaf(dx, dx, eps2);
-				r2 = __builtin_fmaf(dy, dy, r2);
-				r2 = __builtin_fmaf(dz, dz, r2);
-				const float inv = __builtin_amdgcn_rsqf(r2);
-				const float inv2 = inv * inv;
-				const float inv3 = inv2 * inv;
-				const float i5_3 = inv3 * (3.0f * inv2);
-				// two rounded products: exactly zero when j == k
-				const float wx = bk.w * nj.x - bj.w * nk.x;
-				const float wy = bk.w * nj.y - bj.w * nk.y;
-				const float wz = bk.w * nj.z - bj.w * nk.z;
-				const float dw = __builtin_fmaf(dz, wz, __builtin_fmaf(dy, wy, dx * wx));
-				const float q = i5_3 * dw;
-				tx = __builtin_fmaf(-q, dx, __builtin_fmaf(inv3, wx, tx));
-				ty = __builtin_fmaf(-q, dy, __builtin_fmaf(inv3, wy, ty));
-				tz = __builtin_fmaf(-q, dz, __builtin_fmaf(inv3, wz, tz));
-			}
+	unsigned jt = j0;
+	// the equal-mass tiles need m_k = m too: k beyond n_uni (padding, or a
+	// partial tile) takes the general sum throughout.  n_uni is a multiple of
+	// 128 and a wavefront's 64 k are aligned, so the test is the same on every
+	// lane; lane 0's k keeps the branch (and the j stream) scalar.
+	if (__builtin_amdgcn_readfirstlane(k) < n_uni) {
+		for (; jt < ju; jt += kTile) {
+			float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+			vjp_tile<true>(body, nu, jt, j1, nbb, nbn, bk, nk, eps2, tx, ty, tz);
+			kahan_add(Sx, Cx, mu * tx);
+			kahan_add(Sy, Cy, mu * ty);
+			kahan_add(Sz, Cz, mu * tz);
 		}
+	}
+	for (; jt < j1; jt += kTile) {
+		float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+		vjp_tile<false>(body, nu, jt, j1, nbb, nbn, bk, nk, eps2, tx, ty, tz);
 		kahan_add(Sx, Cx, tx);
 		kahan_add(Sy, Cy, ty);
 		kahan_add(Sz, Cz, tz);
