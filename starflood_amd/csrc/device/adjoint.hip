@@ -135,18 +135,34 @@ __global__ __launch_bounds__(kBlock) void k_vjp(const float4* __restrict__ body,
 }
 
 // g_k (= or +=) G * sum over slices of (sum - compensation), in FP64
+// A block takes kCombI particles; its wavefronts take the slices s = w,
+// w + kCombG, ... (w = wavefront) and wavefront 0 adds their sums in order
+// (all lanes of one particle per slice would leave most CUs idle at small N).
+constexpr unsigned kCombI = 64;
+constexpr unsigned kCombG = kBlock / kCombI;
 __global__ __launch_bounds__(kBlock) void k_vjp_combine(const float4* __restrict__ part, unsigned n_pad,
                                                         unsigned slices, unsigned N, double G, int accumulate,
                                                         float* __restrict__ g) {
-	const unsigned k = blockIdx.x * kBlock + threadIdx.x;
-	if (k >= N) return;
+	__shared__ double red[kCombG][3][kCombI];
+	const unsigned kl = threadIdx.x % kCombI, w = threadIdx.x / kCombI;
+	const unsigned k = blockIdx.x * kCombI + kl;   // < n_pad: the grid covers N in kCombI steps
 	double sx = 0.0, sy = 0.0, sz = 0.0;
-	for (unsigned s = 0; s < slices; ++s) {
+	for (unsigned s = w; s < slices; s += kCombG) {
 		const float4 a = part[(size_t)(2u * s) * n_pad + k];
 		const float4 c = part[(size_t)(2u * s + 1u) * n_pad + k];
 		sx += (double)a.x - (double)c.x;
 		sy += (double)a.y - (double)c.y;
 		sz += (double)a.z - (double)c.z;
+	}
+	red[w][0][kl] = sx;
+	red[w][1][kl] = sy;
+	red[w][2][kl] = sz;
+	__syncthreads();
+	if (w != 0 || k >= N) return;
+	for (unsigned q = 1; q < kCombG; ++q) {
+		sx += red[q][0][kl];
+		sy += red[q][1][kl];
+		sz += red[q][2][kl];
 	}
 	float gx = (float)(G * sx), gy = (float)(G * sy), gz = (float)(G * sz);
 	if (accumulate) {
@@ -224,8 +240,8 @@ int sfd_force_vjp(const float* body, const float* mas, const float* mu, unsigned
 	hipLaunchKernelGGL(k_vjp, dim3(grid_v), dim3(kBlock), 0, s, reinterpret_cast<const float4*>(body), nu4, n_pad,
 	                   slices, slice_len, eps2, reinterpret_cast<float4*>(part));
 	sf_prof_end(tok, s);
-	hipLaunchKernelGGL(k_vjp_combine, grid(N), dim3(kBlock), 0, s, reinterpret_cast<const float4*>(part), n_pad,
-	                   slices, N, G, accumulate, g);
+	hipLaunchKernelGGL(k_vjp_combine, dim3((N + kCombI - 1) / kCombI), dim3(kBlock), 0, s,
+	                   reinterpret_cast<const float4*>(part), n_pad, slices, N, G, accumulate, g);
 	return sf_hip_check(hipGetLastError(), "sfd_force_vjp", "launch");
 }
 

@@ -222,21 +222,40 @@ __global__ __launch_bounds__(kBlock) void k_pp_fast(const float4* __restrict__ b
 }
 
 // Fixed-order FP64 combine of the slice partials; Kahan's compensation c
-// holds the negated lost low part, so each slice contributes (s - c).
+// holds the negated lost low part, so each slice contributes (s - c).  A
+// block takes kCombI particles; its wavefronts take the slices s = g, g +
+// kCombG, ... (g = wavefront), each lane one particle (coalesced loads), and
+// wavefront 0 adds the kCombG sums in order.  One lane per particle over all
+// slices left most CUs idle at small N (64 blocks at N = 16384: 20 us).
+constexpr unsigned kCombI = 64;
+constexpr unsigned kCombG = kBlock / kCombI;
 __global__ __launch_bounds__(kBlock) void k_pp_combine(const float4* __restrict__ part, unsigned n_pad,
                                                        unsigned slices, const float4* __restrict__ body,
                                                        unsigned N, double G, int acc_phys,
                                                        float* __restrict__ acc, float* __restrict__ pot) {
-	const unsigned i = blockIdx.x * kBlock + threadIdx.x;
-	if (i >= N) return;
+	__shared__ double red[kCombG][4][kCombI];
+	const unsigned il = threadIdx.x % kCombI, g = threadIdx.x / kCombI;
+	const unsigned i = blockIdx.x * kCombI + il;   // < n_pad: the grid covers N in kCombI steps
 	double sx = 0.0, sy = 0.0, sz = 0.0, sp = 0.0;
-	for (unsigned s = 0; s < slices; ++s) {
+	for (unsigned s = g; s < slices; s += kCombG) {
 		const float4 a = part[(size_t)(2u * s) * n_pad + i];
 		const float4 c = part[(size_t)(2u * s + 1u) * n_pad + i];
 		sx += (double)a.x - (double)c.x;
 		sy += (double)a.y - (double)c.y;
 		sz += (double)a.z - (double)c.z;
 		sp += (double)a.w - (double)c.w;
+	}
+	red[g][0][il] = sx;
+	red[g][1][il] = sy;
+	red[g][2][il] = sz;
+	red[g][3][il] = sp;
+	__syncthreads();
+	if (g != 0 || i >= N) return;
+	for (unsigned q = 1; q < kCombG; ++q) {
+		sx += red[q][0][il];
+		sy += red[q][1][il];
+		sz += red[q][2][il];
+		sp += red[q][3][il];
 	}
 	// sum_j m_j (r_j - r_i)/s^3 = -sum_j m_j r_ij/s^3, so acc = G (m_i) * sum
 	const double f = acc_phys ? G : G * (double)body[i].w;
@@ -403,7 +422,7 @@ extern "C" int sfd_pp_fast(const float* body, unsigned N, unsigned n_pad, float 
 	sf_prof_end(tok, s);
 	if (sf_hip_check(hipGetLastError(), "sfd_pp_fast", "k_pp_fast launch")) return -1;
 	const int tok2 = sf_prof_begin("pp_combine", s);
-	hipLaunchKernelGGL(k_pp_combine, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p4, n_pad, slices, b4, N,
+	hipLaunchKernelGGL(k_pp_combine, dim3((N + kCombI - 1) / kCombI), dim3(kBlock), 0, s, p4, n_pad, slices, b4, N,
 	                   G, acc_phys, acc, pot);
 	sf_prof_end(tok2, s);
 	return sf_hip_check(hipGetLastError(), "sfd_pp_fast", "k_pp_combine launch");
