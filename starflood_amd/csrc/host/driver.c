@@ -38,9 +38,17 @@ static int pattern_ok(const char* p) {
 	return conv == 1;
 }
 
+/* <dir>/<name><ext> as a pattern: a '%' of the directory is doubled, so
+   only the name's conversion formats */
 static int make_pattern(char* out, size_t cap, const char* dir, const char* name, const char* ext) {
-	const int n = snprintf(out, cap, "%s/%s%s", dir, name, ext);
-	return n > 0 && (size_t)n < cap;
+	size_t n = 0;
+	for (const char* c = dir; *c; ++c) {
+		if (n + 2u >= cap) return 0;
+		if (*c == '%') out[n++] = '%';
+		out[n++] = *c;
+	}
+	const int m = snprintf(out + n, cap - n, "/%s%s", name, ext);
+	return m > 0 && (size_t)m < cap - n;
 }
 
 /* snprintf with a pattern checked by pattern_ok */
@@ -83,10 +91,6 @@ int sf_run(const sf_run_config_t* cfg) {
 	const int enable_sim = cfg->enable_sim != 0, enable_vis = cfg->enable_vis != 0;
 	const int sim_io = cfg->sim_filename != NULL, vis_io = cfg->vis_filename != NULL;
 	const unsigned int oi = cfg->output_interval;
-	if (enable_vis && (cfg->vis_sizex == 0u || cfg->vis_sizey == 0u)) {
-		sf_error(fn, "enable_vis needs vis_sizex and vis_sizey");
-		return STARFLOOD_FAILURE;
-	}
 	if ((unsigned int)cfg->vis_file_type >= (unsigned int)NUM_IMAGE_FILETYPE) {
 		sf_error(fn, "unknown vis_file_type %d", (int)cfg->vis_file_type);
 		return STARFLOOD_FAILURE;
@@ -95,7 +99,7 @@ int sf_run(const sf_run_config_t* cfg) {
 	sf_get_params(&p);
 	enum { CAP = 4096 };
 	char sim_pattern[CAP], vis_pattern[CAP], sim_name[CAP], vis_name[CAP], stop_name[CAP];
-	if (!make_pattern(stop_name, CAP, p.out_dir, "stop", "") ||
+	if (snprintf(stop_name, CAP, "%s/stop", p.out_dir) >= (int)CAP ||
 	    (sim_io && !make_pattern(sim_pattern, CAP, p.out_dir, cfg->sim_filename, ".raw")) ||
 	    (vis_io && !make_pattern(vis_pattern, CAP, p.out_dir, cfg->vis_filename, vis_ext(cfg->vis_file_type)))) {
 		sf_error(fn, "output path too long");
@@ -103,6 +107,10 @@ int sf_run(const sf_run_config_t* cfg) {
 	}
 	if ((sim_io && !pattern_ok(sim_pattern)) || (vis_io && !pattern_ok(vis_pattern))) {
 		sf_error(fn, "a file name pattern needs exactly one %%u conversion (flags 0 and -, a width)");
+		return STARFLOOD_FAILURE;
+	}
+	if (enable_vis && (cfg->vis_sizex == 0u || cfg->vis_sizey == 0u)) {
+		sf_error(fn, "enable_vis needs vis_sizex and vis_sizey");
 		return STARFLOOD_FAILURE;
 	}
 

@@ -55,3 +55,22 @@ def test_sf_run_rejects_bad_configurations_before_any_device_work(tmp_path):
     assert lib.sf_run(C.byref(cfg)) == L.STARFLOOD_FAILURE
     assert list(tmp_path.iterdir()) == []
     L.set_params()
+
+
+def test_sf_run_output_directory_percent_is_text(tmp_path):
+    """A '%' in out_dir is doubled in the file-name pattern, so the name's
+    one %u stays the only conversion (no device is used: the pattern checks
+    come first, and the pattern-check error does not appear)."""
+    import ctypes as C
+    from starflood_amd import _lib as L
+    lib = L.load()
+    L.set_params(out_dir=str(tmp_path / "100%u"))
+    cfg = L.RunConfig(num_bodies=16, num_timesteps=1, enable_sim=1, enable_vis=1, vis_sizex=0, vis_sizey=0,
+                      sim_filename=b"sim/step_%04u")
+    # enable_vis without a frame size is checked after the patterns and fails
+    # before any device work: the directory's %u did not fail the pattern
+    assert lib.sf_run(C.byref(cfg)) == L.STARFLOOD_FAILURE
+    assert "vis_sizex" in L.last_error()
+    cfg = L.RunConfig(num_bodies=16, num_timesteps=1, enable_sim=1, sim_filename=b"sim/step_%04u_%u")
+    assert lib.sf_run(C.byref(cfg)) == L.STARFLOOD_FAILURE and "%u" in L.last_error()
+    L.set_params()
