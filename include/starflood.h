@@ -26,9 +26,13 @@
  *   - tree_build errors are propagated by sim_solv instead of being ignored
  *     (reference src/simulation.c:414 drops the return code).
  *
- * SOLVER_VIS (reference config.h:32) is not supported: the library exports
- * the non-SOLVER_VIS prototypes only.  STARFLOOD_DOUBLE_PRECISION is not
- * supported: `real` is binary32.
+ * SOLVER_VIS (reference config.h:32, defined there by default) selects the
+ * prototypes of sim_solv, sim_step and solve_gravity_part_tree that also take
+ * the tree debug overlay's render buffer, as the reference headers do.
+ * libstarflood_hip.so exports the plain forms; libstarflood_hip_solvervis.so
+ * is the same library built with SOLVER_VIS, exporting the overlay forms,
+ * which accept and ignore the buffer (the overlay is not drawn).
+ * STARFLOOD_DOUBLE_PRECISION is not supported: `real` is binary32.
  */
 #ifndef STARFLOOD_H
 #define STARFLOOD_H
@@ -136,6 +140,18 @@ int sim_read_raw(sim_t* SF_RESTRICT sim_ptr, const char* SF_RESTRICT filename);
 /* replaces reference src/simulation.h:85 (src/simulation_io.c:81-149) */
 int sim_save_raw(const sim_t* SF_RESTRICT sim_ptr, const char* SF_RESTRICT filename);
 
+#ifdef SOLVER_VIS
+/* replaces reference src/simulation.h:92 (src/simulation.c:231-345); the
+   render buffer (the tree debug overlay) is ignored */
+int sim_solv(sim_t* SF_RESTRICT sim_ptr, f32* SF_RESTRICT render_buffer, unsigned int sizex, unsigned int sizey);
+
+/* replaces reference src/simulation.h:95 (src/simulation.c:349-519) */
+int sim_step(sim_t* SF_RESTRICT sim_ptr, f32* SF_RESTRICT render_buffer, unsigned int sizex, unsigned int sizey);
+
+/* replaces reference src/gravity.h:15 (src/gravity.c:245-663) */
+int solve_gravity_part_tree(const sim_t* SF_RESTRICT sim_ptr, f32* render_buffer, unsigned int sizex,
+                            unsigned int sizey);
+#else
 /* replaces reference src/simulation.h:98 (src/simulation.c:231-345) */
 int sim_solv(sim_t* SF_RESTRICT sim_ptr);
 
@@ -144,6 +160,7 @@ int sim_step(sim_t* SF_RESTRICT sim_ptr);
 
 /* replaces reference src/gravity.h:17 (src/gravity.c:245-663) */
 int solve_gravity_part_tree(const sim_t* SF_RESTRICT sim_ptr);
+#endif
 
 /* replaces reference src/gravity.h:22 (src/gravity.c:666-819) */
 int solve_gravity_part_part(const sim_t* SF_RESTRICT sim_ptr);

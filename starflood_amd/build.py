@@ -18,6 +18,9 @@ CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB_DIR = PKG / "lib"
 LIB = LIB_DIR / "libstarflood_hip.so"
+# the same library built with SOLVER_VIS: sim_solv, sim_step and
+# solve_gravity_part_tree in the reference's render-buffer prototype form
+LIB_SOLVERVIS = LIB_DIR / "libstarflood_hip_solvervis.so"
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = shutil.which("hipcc") or f"{ROCM}/bin/hipcc"
@@ -61,28 +64,32 @@ def _run(cmd: list[str]) -> None:
 
 def build(verbose: bool = False) -> Path:
     OBJ.mkdir(parents=True, exist_ok=True)
+    (OBJ / "solvervis").mkdir(parents=True, exist_ok=True)
     LIB_DIR.mkdir(parents=True, exist_ok=True)
-    jobs = []
+    dev, host, host_vis = [], [], []
     for src in sorted((CSRC / "device").glob("*.hip")):
         obj = OBJ / (src.stem + ".hip.o")
-        jobs.append(([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)], obj))
+        dev.append(([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)], obj))
     for src in sorted((CSRC / "host").glob("*.c")):
         obj = OBJ / (src.stem + ".c.o")
-        jobs.append(([CC, *C_FLAGS, "-c", str(src), "-o", str(obj)], obj))
+        host.append(([CC, *C_FLAGS, "-c", str(src), "-o", str(obj)], obj))
+        obj_vis = OBJ / "solvervis" / (src.stem + ".c.o")
+        host_vis.append(([CC, *C_FLAGS, "-DSOLVER_VIS", "-c", str(src), "-o", str(obj_vis)], obj_vis))
     workers = max(1, min(8, os.cpu_count() or 1))
     with cf.ThreadPoolExecutor(workers) as ex:
-        futs = [ex.submit(_run, cmd) for cmd, _ in jobs]
+        futs = [ex.submit(_run, cmd) for cmd, _ in dev + host + host_vis]
         for f in futs:
             f.result()
     # export exactly what include/*.h declares; bind internal calls locally
     vmap = OBJ / "exports.map"
     vmap.write_text("{\n  global:\n" + "".join(f"    {n};\n" for n in public_symbols()) + "  local: *;\n};\n")
-    tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *[str(o) for _, o in jobs],
-          "-Wl,--no-undefined", "-Wl,-Bsymbolic", f"-Wl,--version-script={vmap}", "-lm"])
-    os.replace(tmp, LIB)
-    if verbose:
-        print(f"built {LIB}")
+    for lib, objs in ((LIB, dev + host), (LIB_SOLVERVIS, dev + host_vis)):
+        tmp = lib.with_suffix(".so.tmp")
+        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *[str(o) for _, o in objs],
+              "-Wl,--no-undefined", "-Wl,-Bsymbolic", f"-Wl,--version-script={vmap}", "-lm"])
+        os.replace(tmp, lib)
+        if verbose:
+            print(f"built {lib}")
     return LIB
 
 

@@ -142,7 +142,7 @@ int sf_run(const sf_run_config_t* cfg) {
 			rc = STARFLOOD_FAILURE;
 			goto free_sim;
 		}
-		if (STARFLOOD_SUCCESS != sim_solv(&sim)) {
+		if (STARFLOOD_SUCCESS != sf_sim_solv(&sim)) {
 			sf_error(fn, "sim_solv() failed");
 			rc = STARFLOOD_FAILURE;
 			goto free_sim;
@@ -190,7 +190,7 @@ int sf_run(const sf_run_config_t* cfg) {
 		if (output_step && enable_vis && STARFLOOD_SUCCESS != vis_draw(&vis, &sim))
 			fprintf(stderr, "error: vis_draw(&vis, &sim) failed.\n");
 		/* src/main.c:541-550 */
-		if (step < cfg->num_timesteps && enable_sim && STARFLOOD_SUCCESS != sim_step(&sim))
+		if (step < cfg->num_timesteps && enable_sim && STARFLOOD_SUCCESS != sf_sim_step(&sim))
 			fprintf(stderr, "error: sim_step(&sim) failed.\n");
 		/* src/main.c:552-588 */
 		if (output_step && enable_vis && vis_io) {

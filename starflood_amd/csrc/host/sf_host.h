@@ -27,6 +27,10 @@ int sf_tree_check(tree_t* tree, hipStream_t s);
    holds the previous forces: velocities and u are predicted pred_dt ahead. */
 int sf_sph_prepare(sf_ctx* c, float pred_dt);
 int sf_solve_sph(sf_ctx* c);
+/* simulation.c: sim_solv and sim_step, whichever prototype form the library
+   exports (SOLVER_VIS or not) */
+int sf_sim_solv(sim_t* SF_RESTRICT sim_ptr);
+int sf_sim_step(sim_t* SF_RESTRICT sim_ptr);
 /* simulation.c: the solver sequence of sim_solv on device state
    (sph_pred_dt: see sf_sph_prepare) */
 int sf_solve_all(sf_ctx* c, tree_t* tree, unsigned int step_number, float sph_pred_dt);

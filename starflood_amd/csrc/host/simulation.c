@@ -158,7 +158,7 @@ int sf_solve_all(sf_ctx* c, tree_t* tree, unsigned int step_number, float sph_pr
 
 /* reference src/simulation.c:231-345.  A failed tree build fails the call
    (the reference ignores tree_build's return code, :414). */
-int sim_solv(sim_t* SF_RESTRICT sim_ptr) {
+int sf_sim_solv(sim_t* SF_RESTRICT sim_ptr) {
 	if (!sim_ptr) return STARFLOOD_FAILURE;
 	sf_ctx* c = sf_ctx_lookup(sim_ptr->mem, "sim_solv");
 	if (!c) return STARFLOOD_FAILURE;
@@ -176,7 +176,7 @@ static void tev_record(int k, hipStream_t s) {
 }
 
 /* reference src/simulation.c:349-519 */
-int sim_step(sim_t* SF_RESTRICT sim_ptr) {
+int sf_sim_step(sim_t* SF_RESTRICT sim_ptr) {
 	if (!sim_ptr) return STARFLOOD_FAILURE;
 	sf_ctx* c = sf_ctx_lookup(sim_ptr->mem, "sim_step");
 	if (!c) return STARFLOOD_FAILURE;
@@ -336,3 +336,26 @@ int sf_get_stats(const sim_t* sim_ptr, sf_stats_t* out) {
 	out->K = r[10];
 	return STARFLOOD_SUCCESS;
 }
+
+/* The exported entry points, in the prototype form the caller's SOLVER_VIS
+   selects (reference src/simulation.h:90-101).  The SOLVER_VIS form's render
+   buffer is the reference's tree debug overlay, which is not drawn. */
+#ifdef SOLVER_VIS
+int sim_solv(sim_t* SF_RESTRICT sim_ptr, f32* SF_RESTRICT render_buffer, unsigned int sizex, unsigned int sizey) {
+	(void)render_buffer;
+	(void)sizex;
+	(void)sizey;
+	return sf_sim_solv(sim_ptr);
+}
+
+int sim_step(sim_t* SF_RESTRICT sim_ptr, f32* SF_RESTRICT render_buffer, unsigned int sizex, unsigned int sizey) {
+	(void)render_buffer;
+	(void)sizex;
+	(void)sizey;
+	return sf_sim_step(sim_ptr);
+}
+#else
+int sim_solv(sim_t* SF_RESTRICT sim_ptr) { return sf_sim_solv(sim_ptr); }
+
+int sim_step(sim_t* SF_RESTRICT sim_ptr) { return sf_sim_step(sim_ptr); }
+#endif

@@ -335,7 +335,7 @@ int sf_tree_check(tree_t* tree, hipStream_t s) {
 
 /* reference src/gravity.c:245-663: walks the tree last built for this
    simulation (tree_build or sim_solv) and writes acc and pot. */
-int solve_gravity_part_tree(const sim_t* SF_RESTRICT sim_ptr) {
+static int gravity_part_tree(const sim_t* SF_RESTRICT sim_ptr) {
 	if (!sim_ptr) return STARFLOOD_FAILURE;
 	sf_ctx* c = sf_ctx_lookup(sim_ptr->mem, "solve_gravity_part_tree");
 	if (!c) return STARFLOOD_FAILURE;
@@ -363,3 +363,17 @@ int solve_gravity_part_tree(const sim_t* SF_RESTRICT sim_ptr) {
 	if (sf_finish(c, "solve_gravity_part_tree")) rc = -1;
 	return rc ? STARFLOOD_FAILURE : STARFLOOD_SUCCESS;
 }
+
+/* the prototype form the caller's SOLVER_VIS selects (reference
+   src/gravity.h:14-18); the debug overlay is not drawn */
+#ifdef SOLVER_VIS
+int solve_gravity_part_tree(const sim_t* SF_RESTRICT sim_ptr, f32* render_buffer, unsigned int sizex,
+                            unsigned int sizey) {
+	(void)render_buffer;
+	(void)sizex;
+	(void)sizey;
+	return gravity_part_tree(sim_ptr);
+}
+#else
+int solve_gravity_part_tree(const sim_t* SF_RESTRICT sim_ptr) { return gravity_part_tree(sim_ptr); }
+#endif

@@ -31,6 +31,39 @@ def test_library_exports_exactly_the_header_declarations():
         assert getattr(lib, n) is not None
 
 
+VIS_PROTO_PROBE = r"""
+#include "starflood.h"
+#ifdef SOLVER_VIS
+/* the reference's prototypes, src/simulation.h:92,95 and src/gravity.h:15 */
+int (*p_solv)(sim_t* restrict, f32* restrict, unsigned int, unsigned int) = sim_solv;
+int (*p_step)(sim_t* restrict, f32* restrict, unsigned int, unsigned int) = sim_step;
+int (*p_tree)(const sim_t* restrict, f32*, unsigned int, unsigned int) = solve_gravity_part_tree;
+#else
+/* src/simulation.h:98,101 and src/gravity.h:17 */
+int (*p_solv)(sim_t* restrict) = sim_solv;
+int (*p_step)(sim_t* restrict) = sim_step;
+int (*p_tree)(const sim_t* restrict) = solve_gravity_part_tree;
+#endif
+int main(void) { return 0; }
+"""
+
+
+@pytest.mark.parametrize("solver_vis", [False, True])
+def test_solver_vis_prototypes_and_library(solver_vis):
+    """Both prototype forms of the reference compile against include/ (a
+    mismatch is an error), and the SOLVER_VIS library exports the same names."""
+    from starflood_amd.build import LIB_SOLVERVIS
+    with tempfile.TemporaryDirectory() as d:
+        src = Path(d) / "p.c"
+        src.write_text(VIS_PROTO_PROBE)
+        flags = ["-DSOLVER_VIS"] if solver_vis else []
+        r = subprocess.run(["gcc", "-std=c99", "-Werror=incompatible-pointer-types", *flags, f"-I{REPO / 'include'}",
+                            "-c", str(src), "-o", str(Path(d) / "p.o")], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+    if solver_vis:
+        assert exported(LIB_SOLVERVIS) == set(public_symbols())
+
+
 LAYOUT_PROBE = r"""
 #include <stdio.h>
 #include <stddef.h>

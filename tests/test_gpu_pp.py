@@ -48,6 +48,37 @@ def test_config1_100_steps_bit_identical_in_step_order():
         assert np.array_equal(bits(s.mem), bits(g["final"]))
 
 
+def test_solver_vis_library_reproduces_the_reference_steps():
+    """libstarflood_hip_solvervis.so (the SOLVER_VIS prototypes, render
+    buffer ignored) gives the reference's bits, as the plain library does."""
+    import ctypes as C
+    import os
+    from starflood_amd.build import LIB_SOLVERVIS
+    g = golden("f1_pp_n256.npz")
+    lib = C.CDLL(str(LIB_SOLVERVIS), mode=os.RTLD_NOW | C.RTLD_LOCAL)
+    vis_sig = [C.POINTER(L.SimT), C.c_void_p, C.c_uint, C.c_uint]
+    lib.sim_solv.argtypes = lib.sim_step.argtypes = vis_sig
+    lib.sim_init.argtypes = [C.POINTER(L.SimT), C.c_uint]
+    lib.sim_free.argtypes = [C.POINTER(L.SimT)]
+    lib.sf_params_default.argtypes = lib.sf_set_params.argtypes = [C.POINTER(L.Params)]
+    p = L.Params()
+    lib.sf_params_default(C.byref(p))
+    p.enable_tree, p.epsilon, p.dt, p.logging = 0, 1e-2, 1e-3, 0
+    assert lib.sf_set_params(C.byref(p)) == 0
+    s = L.SimT()
+    assert lib.sim_init(C.byref(s), 256) == 0
+    mem = np.ctypeslib.as_array((C.c_float * (256 * 13)).from_address(s.mem))
+    mem[:] = g["ic"]
+    frame = (C.c_float * (4 * 8 * 8))()
+    assert lib.sim_solv(C.byref(s), frame, 8, 8) == 0
+    assert np.array_equal(bits(mem), bits(g["mem0"]))
+    for n in range(10):
+        s.step_number = n
+        assert lib.sim_step(C.byref(s), frame, 8, 8) == 0
+    assert np.array_equal(bits(mem), bits(g["mem10"]))
+    assert lib.sim_free(C.byref(s)) == 0
+
+
 def test_exact_resident_mode_matches_mirror_mode():
     g = golden("f1_pp_n256.npz")
     with exact_sim(256, mem_mode=L.SF_MEM_RESIDENT) as s:
