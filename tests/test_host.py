@@ -74,3 +74,17 @@ def test_sf_run_output_directory_percent_is_text(tmp_path):
     cfg = L.RunConfig(num_bodies=16, num_timesteps=1, enable_sim=1, sim_filename=b"sim/step_%04u_%u")
     assert lib.sf_run(C.byref(cfg)) == L.STARFLOOD_FAILURE and "%u" in L.last_error()
     L.set_params()
+
+
+def test_sim_init_fails_without_the_output_directory(tmp_path):
+    """As the reference (src/simulation.c:58-76): with logging on, sim_init
+    opens <out_dir>/statistics.csv and timings_sim_step.csv first and fails
+    when the directory is missing (before any device work)."""
+    import ctypes as C
+    from starflood_amd import _lib as L
+    lib = L.load()
+    L.set_params(logging=1, out_dir=str(tmp_path / "missing"))
+    s = L.SimT()
+    assert lib.sim_init(C.byref(s), 16) == L.STARFLOOD_FAILURE
+    assert not (tmp_path / "missing").exists()
+    L.set_params()
