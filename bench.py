@@ -132,6 +132,34 @@ def cpu_baseline(budget_s: float = 12.0) -> dict:
         out["reference_tree_solve"] = {"N": REFTREE_N, "ms": t * 1e3,
                                        "mem_sha256": hashlib.sha256(r.mem.tobytes()).hexdigest()}
         r.close()
+    if O.ref_available(BH4M_VARIANT):
+        # the reference's own Barnes-Hut solve (sim_solv: mean-radius bounds,
+        # serial octree, OpenMP walk; TREE_NODES_MAX 2^23) of the 4M Plummer
+        # state of bh_4m (initial state), and its force error against the FP64
+        # direct sum on every 1024th particle
+        N = 4194304
+        s = Simulation(N, enable_tree=0, logging=0, epsilon=EPS, acc_mode=L.SF_ACC_PHYSICAL,
+                       mem_mode=L.SF_MEM_RESIDENT)
+        s.ic_plummer(seed=21, first=0, count=N)
+        idx = np.arange(0, N, 1024, dtype=np.uint32)
+        ref, _ = s.direct_f64(idx)
+        pos, mas = s.pos.copy(), s.mas.copy()
+        s.close()
+        L.set_params()
+        r = O.RefSim(BH4M_VARIANT, N)
+        r.block(0, 3)[:] = pos.ravel()
+        r.block(9)[:] = mas
+        r.sim.step_number = 0
+        t0 = time.perf_counter()
+        rc = r.solve()
+        secs = time.perf_counter() - t0
+        acc = r.block(6, 3).reshape(N, 3)[idx].astype(np.float64) / mas[idx, None].astype(np.float64)
+        out["reference_bh_4m"] = {"N": N, "theta": 0.5, "solve_s": secs, "rc": int(rc),
+                                  "num_nodes": _printed_nodes(r.last_stdout),
+                                  "force_err_vs_fp64": rel_err_pcts(acc, ref),
+                                  "sample": "initial state of bh_4m, every 1024th particle",
+                                  "criterion": "the reference's: open iff bodies > 8 and (mean edge / distance)^2 >= theta"}
+        r.close()
     if O.ref_program_available(RUN_PROGRAM):
         # the reference program's own run of the sf_run workload
         import tempfile
@@ -239,6 +267,16 @@ def bench_vis(reps: int = 5) -> dict:
     s.close()
     return out
 
+
+BH4M_VARIANT = "tree4m_e2e-3_dt1e-3"   # reference build: eps = 0.002, dt = 1e-3, TREE_NODES_MAX = 2^23
+
+
+def _printed_nodes(stdout: str):
+    """The node count tree_build prints ("built tree with %u nodes",
+    src/tree.c:315); it keeps the count in a local copy of tree_t."""
+    import re
+    m = re.search(r"built tree with (\d+) nodes", stdout or "")
+    return int(m.group(1)) if m else None
 
 RUN_N = 16384
 RUN_STEPS = 10
@@ -619,6 +657,10 @@ def main():
         ref_vis = out["cpu_baseline"].get("reference_vis_draw")
         if ref_vis and "extra" in out:
             out["extra"]["vis_65k"]["reference_cpu_draw_ms"] = ref_vis["ms"]
+        ref_bh = out["cpu_baseline"].get("reference_bh_4m")
+        if ref_bh and "extra" in out:
+            out["extra"]["bh_4m"]["reference_cpu_solve_s"] = ref_bh["solve_s"]
+            out["extra"]["bh_4m"]["reference_cpu_force_err_vs_fp64"] = ref_bh["force_err_vs_fp64"]
         ref_run = out["cpu_baseline"].get("reference_run")
         if ref_run and "extra" in out:
             rr = out["extra"]["run_16k"]
