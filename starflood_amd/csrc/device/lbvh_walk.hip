@@ -65,13 +65,16 @@ __device__ __forceinline__ void fold(Acc& a) {
 	a.tx = a.ty = a.tz = a.tp = 0.0f;
 }
 
+// UNIFORM: every q has the same mass, left out of the sums (the caller
+// scales the partial by it before the fold)
+template <bool UNIFORM = false>
 __device__ __forceinline__ void interact(Acc& a, float xi, float yi, float zi, float4 bj, float eps2) {
 	const float dx = bj.x - xi, dy = bj.y - yi, dz = bj.z - zi;
 	float r2 = __builtin_fmaf(dx, dx, eps2);
 	r2 = __builtin_fmaf(dy, dy, r2);
 	r2 = __builtin_fmaf(dz, dz, r2);
 	const float inv = __builtin_amdgcn_rsqf(r2);
-	const float mi = bj.w * inv;
+	const float mi = UNIFORM ? inv : bj.w * inv;
 	const float w = mi * (inv * inv);
 	a.tx = __builtin_fmaf(w, dx, a.tx);
 	a.ty = __builtin_fmaf(w, dy, a.ty);
@@ -80,12 +83,20 @@ __device__ __forceinline__ void interact(Acc& a, float xi, float yi, float zi, f
 }
 
 // Interactions with q[0 .. n), n <= 64, read from LDS by every lane.
-__device__ __forceinline__ void eval_list(Acc& a, float xi, float yi, float zi, const float4* q, int n, float eps2) {
+template <bool UNIFORM = false>
+__device__ __forceinline__ void eval_list(Acc& a, float xi, float yi, float zi, const float4* q, int n, float eps2,
+                                          float m = 1.0f) {
 	if (n == 64) {
 #pragma unroll 8
-		for (int k = 0; k < 64; ++k) interact(a, xi, yi, zi, q[k], eps2);
+		for (int k = 0; k < 64; ++k) interact<UNIFORM>(a, xi, yi, zi, q[k], eps2);
 	} else {
-		for (int k = 0; k < n; ++k) interact(a, xi, yi, zi, q[k], eps2);
+		for (int k = 0; k < n; ++k) interact<UNIFORM>(a, xi, yi, zi, q[k], eps2);
+	}
+	if (UNIFORM) {
+		a.tx *= m;
+		a.ty *= m;
+		a.tz *= m;
+		a.tp *= m;
 	}
 	fold(a);
 }
@@ -158,6 +169,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
 	float4* part = s_part[w];
 	const unsigned long long lt = (1ull << lane) - 1ull;
 	const int n_nodes = 2 * N - 1;
+	const float m0 = body[0].w;   // the mass of the equal-mass leaf batches
 	const unsigned budget = 2u * (unsigned)n_nodes + 64u;
 	unsigned long long n_cell_tot = 0, n_part_tot = 0, n_batch = 0;
 	unsigned bad = 0;
@@ -304,9 +316,19 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
 						wave_fence();
 						if (npid >= 64 || (drain && q == nl && npid > 0)) {
 							const int take = min(npid, 64);
-							if ((int)lane < take) part[lane] = body[pid[lane]];
+							float pm = m0;
+							if ((int)lane < take) {
+								const float4 pb = body[pid[lane]];
+								part[lane] = pb;
+								pm = pb.w;
+							}
 							wave_fence();
-							eval_list(a, xi, yi, zi, part, take, eps2);
+							// particles of one mass (the common case) leave the
+							// per-pair mass product out
+							if (__ballot(__float_as_uint(pm) != __float_as_uint(m0)) == 0ull)
+								eval_list<true>(a, xi, yi, zi, part, take, eps2, m0);
+							else
+								eval_list(a, xi, yi, zi, part, take, eps2);
 							n_part += (unsigned long long)take;
 							wave_fence();
 							if ((int)lane < npid - take) pid[lane] = pid[take + (int)lane];
