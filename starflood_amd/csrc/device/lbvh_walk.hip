@@ -141,8 +141,9 @@ __device__ __forceinline__ void eval_list_quad(Acc& a, float xi, float yi, float
 	fold(a);
 }
 
-// 5 wavefronts per SIMD: the register budget that keeps the walk spill-free
-// (88 VGPRs) while raising occupancy from 4 (measured 7% faster at 4M).
+// 5 wavefronts per SIMD (96 VGPRs): the monopole walk fits without spills
+// and runs 7% faster at 4M than at 4 per SIMD; the quadrupole walk spills 20
+// VGPRs and is still 3.4% faster than at 4 per SIMD without spills.
 // QUAD: cells carry traceless quadrupoles (node_q0, node_q1).
 template <bool QUAD>
 __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ body, const unsigned* __restrict__ order,
