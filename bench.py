@@ -583,8 +583,8 @@ def main():
                     help="run one secondary workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design "
-                                   "(see DESIGN.md), not a run", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the library runs on one GPU; its sharding across GPUs is designed, not "
+                                   "implemented", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.only:
         fn = {"bh": lambda: bench_bh(steps=args.steps, pp_compare=False), "adjoint": lambda: bench_adjoint(reps=1),

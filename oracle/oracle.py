@@ -278,7 +278,7 @@ def ref_vis_save(render, filetype: int, path=None):
     3 bytes per pixel and the PFM path writes 12 (src/visualization.c:76-87,
     src/visualization_io.c:195-204), past the end of the allocation."""
     if filetype == 0:
-        raise ValueError("the reference's PFM path overflows its buffer in this build")
+        raise ValueError("the PFM path of the reference's PNG-format configuration overflows its buffer")
     lib = C.CDLL(str(REF_DIR / "libref_pp_e1e-2_dt1e-3.so"))
     render = np.ascontiguousarray(render, np.float32)
     h, w = render.shape[:2]

@@ -67,7 +67,7 @@ def stats_rows(csv: str) -> np.ndarray:
 
 
 def run_steps(r: O.RefSim, nsteps: int, keep=()):
-    """sim_step nsteps times with the driver's step numbering
+    """sim_step nsteps times with the reference program's step numbering
     (src/main.c:451: step_number = step index before the step)."""
     kept = {}
     TV = []
