@@ -223,7 +223,7 @@ def bench_reftree(reps: int = 5) -> dict:
 
 
 VIS_N = 65536
-VIS_SIZE = 1080   # the reference driver's frame (src/main.c:81)
+VIS_SIZE = 1080   # the reference program's frame (src/main.c:81)
 
 
 def vis_state():

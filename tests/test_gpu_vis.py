@@ -40,7 +40,7 @@ def test_vis_draw_and_files_bit_identical_to_reference(tmp_path, mem_mode):
 
 
 def test_large_frame_is_deterministic_and_tone_mapped_from_its_counts():
-    """Plummer sphere N = 65536 at 1080 x 1080 (the reference driver's frame):
+    """Plummer sphere N = 65536 at 1080 x 1080 (the reference program's frame):
     two draws are identical; every sample lands once (alpha sums to at most
     128 N); colour counts stay within 100 x alpha; the render buffer is
     tanh(0.01 / 128 * 2^-1 * counts) to 1 ulp (double tanh on the device and
