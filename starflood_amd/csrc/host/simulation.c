@@ -5,6 +5,7 @@
  */
 #define _POSIX_C_SOURCE 200112L
 
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -12,6 +13,11 @@
 #include "sf_host.h"
 
 #define SF_ALIGNMENT 4194304u   /* STARFLOOD_ALIGNMENT, config.h:230 */
+/* smaller sim.mem stays ordinary memory: copies from page-locked memory go
+   through the DMA engine, whose per-copy latency costs more than its rate
+   gains on a few-MB state (N = 65536: reference-tree solve 3.2 -> 4.0 ms),
+   while a 13.6 MB state (N = 262144) steps 2.7 % faster in mirror mode */
+#define SF_PIN_MIN_BYTES 8388608u
 
 /* statistics.csv and timings_sim_step.csv (src/simulation.c:24-30,58-76) */
 static FILE* g_log_statistics = NULL;
@@ -73,21 +79,34 @@ int sim_init(sim_t* SF_RESTRICT sim_ptr, unsigned int N) {
 	sim_t sim = *sim_ptr;
 	const unsigned int dof = sf_sim_dof();
 	const size_t mem_size = sizeof(real) * (size_t)N * (size_t)dof;
+	/* page-locked (the mirror-mode copies then run at full PCIe rate), still
+	   SF_ALIGNMENT aligned inside the allocation; ordinary aligned memory,
+	   as the reference's, for small systems or when pinning fails */
 	void* mem = NULL;
-	if (posix_memalign(&mem, (size_t)SF_ALIGNMENT, mem_size ? mem_size : 1u) != 0 || !mem) {
-		fprintf(stderr, "%s error: mem is NULL after posix_memalign(&mem, %zu, %zu)\n", "sim_init()",
-		        (size_t)SF_ALIGNMENT, mem_size);
-		close_logs();
-		return STARFLOOD_FAILURE;
+	void* pinned = NULL;
+	if (mem_size >= SF_PIN_MIN_BYTES &&
+	    hipHostMalloc(&pinned, mem_size + SF_ALIGNMENT, hipHostMallocDefault) == hipSuccess && pinned) {
+		mem = (void*)(((uintptr_t)pinned + (SF_ALIGNMENT - 1u)) & ~(uintptr_t)(SF_ALIGNMENT - 1u));
+	} else {
+		(void)hipGetLastError();
+		pinned = NULL;
+		if (posix_memalign(&mem, (size_t)SF_ALIGNMENT, mem_size ? mem_size : 1u) != 0 || !mem) {
+			fprintf(stderr, "%s error: mem is NULL after posix_memalign(&mem, %zu, %zu)\n", "sim_init()",
+			        (size_t)SF_ALIGNMENT, mem_size);
+			close_logs();
+			return STARFLOOD_FAILURE;
+		}
 	}
 	memset(mem, 0, mem_size);
 
 	sf_ctx* c = sf_ctx_create(mem, N, dof);
 	if (!c) {
-		free(mem);
+		if (pinned) hipHostFree(pinned);
+		else free(mem);
 		close_logs();
 		return STARFLOOD_FAILURE;
 	}
+	c->host_pinned_base = pinned;
 
 	if (p->verbose) {
 		printf("Simulation Memory Addresses:\n  mem: %p (%zu bytes)\n\n", mem, mem_size);
@@ -111,11 +130,16 @@ int sim_free(sim_t* SF_RESTRICT sim_ptr) {
 	if (!sim_ptr) return STARFLOOD_FAILURE;
 	close_logs();
 	sim_t sim = *sim_ptr;
+	void* pinned = NULL;
 	if (sim.mem) {
 		sf_ctx* c = sf_ctx_lookup(sim.mem, "sim_free");
-		if (c) sf_ctx_destroy(c);
+		if (c) {
+			pinned = c->host_pinned_base;
+			sf_ctx_destroy(c);   /* waits for the context's stream */
+		}
 	}
-	free(sim.mem);
+	if (pinned) hipHostFree(pinned);
+	else free(sim.mem);
 	if (sim.tree.node) tree_free(&sim.tree);
 	sim.step_number = 0u;
 	sim.N = 0u;

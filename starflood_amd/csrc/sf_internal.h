@@ -35,6 +35,7 @@ typedef struct sf_tree_dev sf_tree_dev;   /* host/tree.c */
 
 typedef struct sf_ctx {
 	void* host_mem;          /* key: sim_t.mem */
+	void* host_pinned_base;  /* the page-locked allocation holding sim_t.mem, or NULL (ordinary memory) */
 	unsigned int N;
 	unsigned int dof;        /* reals per particle in sim.mem */
 	unsigned int n_pad;      /* N rounded up to the PP j tile */
