@@ -128,7 +128,9 @@ typedef struct {
 /* replaces reference src/simulation.h:70 (src/simulation.c:33-53) */
 real* sim_find(const sim_t* SF_RESTRICT sim_ptr, enum sim_conf parameter_enum);
 
-/* replaces reference src/simulation.h:73 (src/simulation.c:55-187) */
+/* replaces reference src/simulation.h:73 (src/simulation.c:55-187).
+   sim.mem is 4 MiB aligned and zeroed as in the reference; from 8 MiB up it
+   is page-locked host memory (free it only through sim_free). */
 int sim_init(sim_t* SF_RESTRICT sim_ptr, unsigned int N);
 
 /* replaces reference src/simulation.h:76 (src/simulation.c:189-228) */
