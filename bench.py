@@ -640,6 +640,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": PP_FAST_TRAFFIC_BYTES,
                      "kernel": "k_pp_fast", "kernel_ms": pp_avg_s * 1e3,
+                     "peak_source": "MI355X FP32 157.3 TFLOP/s; the kernel is FP32 VALU work (rsqrt and FMA, no MFMA)",
                      "flops_per_launch": FLOPS_PER_INTERACTION * inter},
         "energy_check": {"K": st["K"], "W": st["W"], "T_ken": st["T"]},
     }
