@@ -165,6 +165,47 @@ def test_fast_pp_equal_mass_tiles_close_to_fp64(oracle, N, masses):
     L.set_params()
 
 
+def test_pack_state_stays_consistent_over_many_launches():
+    """The pack kernel's equal-mass detection keeps its block-count words in
+    device memory and returns them to zero itself.  After hundreds of packs,
+    and after a mass changes mid-run (the general sum must take over, then
+    the equal-mass tiles again), the force equals a fresh context's on the
+    same state bit for bit.  The masses are not a power of two, so the two
+    sums round differently and a stale decision would show."""
+    N = 4096
+    rng = np.random.default_rng(77)
+    kw = dict(precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL, mem_mode=L.SF_MEM_RESIDENT)
+
+    def fresh_force(mem):
+        with exact_sim(N, **kw) as b:
+            b.mem[:] = mem
+            b.to_device()
+            b.solve_gravity_pp()
+            b.to_host()
+            return b.acc.copy(), b.pot.copy()
+
+    with exact_sim(N, **kw) as a:
+        a.pos[:] = rng.normal(size=(N, 3)).astype(np.float32)
+        a.mas[:] = np.float32(1.0 / 3.0 / N)
+        a.to_device()
+        a.solve()
+        for phase in ("equal", "one_differs", "equal_again"):
+            if phase != "equal":
+                a.to_host()
+                a.mas[7] = np.float32(1.0 / 3.0 / N) * (np.float32(1.25) if phase == "one_differs" else 1)
+                a.to_device()
+            for _ in range(150):
+                a.step()
+            a.to_host()
+            state = a.mem.copy()
+            a.solve_gravity_pp()
+            a.to_host()
+            acc_b, pot_b = fresh_force(state)
+            assert np.array_equal(bits(a.acc.ravel()), bits(acc_b.ravel())), phase
+            assert np.array_equal(bits(a.pot), bits(pot_b)), phase
+    L.set_params()
+
+
 def test_f64_direct_sum_kernel_matches_cpu_oracle(oracle):
     N = 8192
     with exact_sim(N, epsilon=2e-3, acc_mode=L.SF_ACC_PHYSICAL) as s:
