@@ -480,14 +480,16 @@ def bench_copy(mib: int = 1024) -> dict:
     return {"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4, "launches": 3}
 
 
-def bench_adjoint(reps: int = 3) -> dict:
+def bench_adjoint(reps: int = 3, ckpt: int = 20) -> dict:
     """Adjoint KDK on two Plummer spheres: N = 16384 (2 x 8192, a = 0.5,
     centres (+-1.5, 0, 0), approach speed 0.3), eps = 0.01, dt = 1e-3, 200
-    steps, checkpoints every 20; loss = mean((x_N - x_target)^2), target run
-    from ICs perturbed by N(0, 1e-3).  Reports forward and backward times."""
+    steps, a checkpoint every `ckpt` steps (20; 1 keeps every step's state,
+    157 MB, and the backward recomputes nothing); loss = mean((x_N -
+    x_target)^2), target run from ICs perturbed by N(0, 1e-3).  Reports
+    forward and backward times."""
     import torch
     from starflood_amd import adjoint as A
-    N, K, k = 16384, 200, 20
+    N, K, k = 16384, 200, ckpt
     s = Simulation(N, enable_tree=0, logging=0, epsilon=1e-2, dt=1e-3)
     s.ic_plummer(seed=31, first=0, count=N // 2, a=0.5, M=0.5, centre=(-1.5, 0, 0), bulk_v=(0.15, 0, 0))
     s.ic_plummer(seed=32, first=N // 2, count=N // 2, a=0.5, M=0.5, centre=(1.5, 0, 0), bulk_v=(-0.15, 0, 0))
@@ -516,7 +518,7 @@ def bench_adjoint(reps: int = 3) -> dict:
         fwd.append(t1 - t0)
         bwd.append(t2 - t1)
     f, b = float(np.median(fwd[1:])), float(np.median(bwd[1:]))
-    return {"workload": "two Plummer spheres N=16384, eps=0.01, dt=1e-3, 200 KDK steps, checkpoint every 20, "
+    return {"workload": f"two Plummer spheres N=16384, eps=0.01, dt=1e-3, 200 KDK steps, checkpoint every {k}, "
                         "MSE loss on final positions",
             "forward_ms": f * 1e3, "backward_ms": b * 1e3, "backward_over_forward": b / f,
             "loss": float(loss.item()), "grad_norm_x0": float(tx.grad.norm().item()),
@@ -587,7 +589,7 @@ def main():
                                    "implemented", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.only:
-        fn = {"bh": lambda: bench_bh(steps=args.steps, pp_compare=False), "adjoint": lambda: bench_adjoint(reps=1),
+        fn = {"bh": lambda: bench_bh(steps=args.steps, pp_compare=False), "adjoint": lambda: {"ckpt20": bench_adjoint(reps=1), "ckpt1": bench_adjoint(reps=1, ckpt=1)},
               "sph": lambda: bench_sph(steps=args.steps),
               "sphgrid": lambda: bench_sph(steps=args.steps, search=L.SF_SPH_SEARCH_GRID), "copy": bench_copy,
               "bh1m": bench_bh1m,
@@ -645,7 +647,8 @@ def main():
         "energy_check": {"K": st["K"], "W": st["W"], "T_ken": st["T"]},
     }
     if not args.no_extra:
-        out["extra"] = {"bh_4m": bench_bh(), "adjoint_16k": bench_adjoint(), "sph_65k": bench_sph(),
+        out["extra"] = {"bh_4m": bench_bh(), "adjoint_16k": bench_adjoint(),
+                        "adjoint_16k_ckpt1": bench_adjoint(ckpt=1), "sph_65k": bench_sph(),
                         "sph_65k_grid": bench_sph(search=L.SF_SPH_SEARCH_GRID), "reftree_65k": bench_reftree(),
                         "vis_65k": bench_vis(), "run_16k": bench_run(), "pp_262k_mirror": bench_pp_mirror()}
     if not args.no_cpu_baseline:

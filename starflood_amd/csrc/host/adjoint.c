@@ -190,7 +190,10 @@ int sf_adjoint_forward(unsigned int N, const float* x0, const float* v0, const f
 	return STARFLOOD_SUCCESS;
 }
 
-/* positions x_{start} .. x_{start+len} of a segment into t->seg */
+/* positions x_{start} .. x_{start+len-1} of a segment into t->seg.  The
+   backward reads x_{start+len} only in the last segment, which the forward
+   left on the tape and is never recomputed, so len - 1 steps suffice (none
+   with a checkpoint every step). */
 static int recompute(sf_tape* t, unsigned int c, unsigned int len, hipStream_t s) {
 	const size_t n3 = 3u * (size_t)t->N, f3 = n3 * sizeof(float);
 	int rc = 0;
@@ -199,7 +202,7 @@ static int recompute(sf_tape* t, unsigned int c, unsigned int len, hipStream_t s
 	rc |= hipMemcpyAsync(t->a, t->ck_a + c * n3, f3, hipMemcpyDeviceToDevice, s) != hipSuccess;
 	rc |= sfd_f64_to_f32(t->x64, (unsigned int)n3, t->x, s);
 	rc |= hipMemcpyAsync(t->seg, t->x, f3, hipMemcpyDeviceToDevice, s) != hipSuccess;
-	for (unsigned int k = 1; k <= len && !rc; ++k) {
+	for (unsigned int k = 1; k < len && !rc; ++k) {
 		rc |= kdk(t, s);
 		rc |= hipMemcpyAsync(t->seg + k * n3, t->x, f3, hipMemcpyDeviceToDevice, s) != hipSuccess;
 	}
