@@ -207,8 +207,8 @@ int sf_pp_direct_f64(const sim_t* sim_ptr, const unsigned int* idx, unsigned int
 /* --- adjoint of the force and of the KDK step (device pointers) ---
    State: x, v are 3N floats (AoS), m is N floats, all device memory.
    The tape keeps (x, v, a) every `ckpt_every` steps on the device
-   (40 N bytes each), and the backward recomputes ckpt_every - 1 steps per
-   segment from them: ckpt_every = 1 trades 40 N bytes per step for no
+   (48 N bytes each: x in FP64), and the backward recomputes ckpt_every - 1 steps per
+   segment from them: ckpt_every = 1 trades 48 N bytes per step for no
    recomputation. */
 typedef struct sf_tape sf_tape;
 /* Run `nsteps` KDK steps of PP gravity from (x0, v0) with the current
