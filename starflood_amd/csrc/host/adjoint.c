@@ -190,23 +190,27 @@ int sf_adjoint_forward(unsigned int N, const float* x0, const float* v0, const f
 	return STARFLOOD_SUCCESS;
 }
 
-/* positions x_{start} .. x_{start+len-1} of a segment into t->seg.  The
-   backward reads x_{start+len} only in the last segment, which the forward
-   left on the tape and is never recomputed, so len - 1 steps suffice (none
-   with a checkpoint every step). */
+/* The positions of segment c that the backward reads, into t->seg:
+   x_{start} .. x_{start+len-1}, and x_{start+len} too for the last segment.
+   Only the last segment reads its end position, so the others take len - 1
+   steps (none with a checkpoint every step).  The forward leaves the last
+   segment on the tape; it is recomputed only when the tape's backward runs
+   again. */
 static int recompute(sf_tape* t, unsigned int c, unsigned int len, hipStream_t s) {
 	const size_t n3 = 3u * (size_t)t->N, f3 = n3 * sizeof(float);
+	const unsigned int steps = c + 1u == t->n_ckpt ? len : len - 1u;
 	int rc = 0;
+	t->seg_first = c * t->ckpt;
+	if (steps == 0u) return sfd_f64_to_f32(t->ck_x64 + c * n3, (unsigned int)n3, t->seg, s);
 	rc |= hipMemcpyAsync(t->x64, t->ck_x64 + c * n3, n3 * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess;
 	rc |= hipMemcpyAsync(t->v, t->ck_v + c * n3, f3, hipMemcpyDeviceToDevice, s) != hipSuccess;
 	rc |= hipMemcpyAsync(t->a, t->ck_a + c * n3, f3, hipMemcpyDeviceToDevice, s) != hipSuccess;
 	rc |= sfd_f64_to_f32(t->x64, (unsigned int)n3, t->x, s);
 	rc |= hipMemcpyAsync(t->seg, t->x, f3, hipMemcpyDeviceToDevice, s) != hipSuccess;
-	for (unsigned int k = 1; k < len && !rc; ++k) {
+	for (unsigned int k = 1; k <= steps && !rc; ++k) {
 		rc |= kdk(t, s);
 		rc |= hipMemcpyAsync(t->seg + k * n3, t->x, f3, hipMemcpyDeviceToDevice, s) != hipSuccess;
 	}
-	t->seg_first = c * t->ckpt;
 	return rc;
 }
 

@@ -139,6 +139,27 @@ def test_checkpoint_interval_does_not_change_result():
             assert np.array_equal(a, b)
 
 
+
+def test_backward_twice_on_one_tape():
+    """A second backward on the same tape (autograd's retain_graph)
+    recomputes the last segment, whose end position x_N the first backward
+    read from the forward's tape: both gradients are bit-identical, for a
+    checkpoint every step, a non-divisor and the whole run."""
+    L.set_params(epsilon=EPS, dt=DT, G=1.0, acc_mode=L.SF_ACC_PHYSICAL)
+    N, K = 700, 30
+    x0, v0, m = two_spheres(N, 11)
+    gxN = dev(np.random.default_rng(12).normal(size=x0.shape))
+    gvN = dev(np.random.default_rng(13).normal(size=x0.shape))
+    for ckpt in (1, 7, K):
+        tx, tv = dev(x0).requires_grad_(True), dev(v0).requires_grad_(True)
+        xN, vN = A.kdk(tx, tv, dev(m), K, ckpt)
+        torch.autograd.backward((xN, vN), (gxN, gvN), retain_graph=True)
+        first = (tx.grad.clone(), tv.grad.clone())
+        tx.grad, tv.grad = None, None
+        torch.autograd.backward((xN, vN), (gxN, gvN))
+        assert torch.equal(tx.grad, first[0]), ckpt
+        assert torch.equal(tv.grad, first[1]), ckpt
+
 def test_matches_fp64_torch_autograd_of_plain_forward():
     """Reference gradient: autograd through a plain FP64 torch KDK (CPU),
     with both the cotangent on x_N and on v_N."""
