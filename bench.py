@@ -325,7 +325,7 @@ def rel_err_pcts(acc, ref):
 
 BH_RUNS = {
     # monopole cells: the reference tree's expansion
-    "monopole_theta0.5": dict(theta=0.5, leaf_size=4, quadrupole=0),
+    "monopole_theta0.5": dict(theta=0.5, leaf_size=3, quadrupole=0),
     # quadrupole cells at the theta whose 99th-percentile force error is below 1e-3
     "quadrupole_theta0.55": dict(theta=0.55, leaf_size=8, quadrupole=1),
 }

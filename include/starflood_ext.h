@@ -94,7 +94,7 @@ typedef struct {
 	double theta;                /* TREE_THETA, config.h:44 (0.5) */
 	unsigned int tree_nodes_max; /* TREE_NODES_MAX, config.h:41 (262144) */
 	unsigned int tree_depth_max; /* TREE_DEPTH_MAX, config.h:38 (16, at most 16) */
-	unsigned int leaf_size;      /* SF_TREE_LBVH leaf capacity (4) */
+	unsigned int leaf_size;      /* SF_TREE_LBVH leaf capacity (3) */
 	int enable_tree;             /* ENABLE_TREE, config.h:35 (1) */
 	int enable_sph;              /* ENABLE_SPH, config.h:190 (0) */
 	int kahan_solver;            /* ENABLE_KAHAN_SUMMATION_SOLVER, config.h:194 (1) */
