@@ -24,7 +24,7 @@
  *   - sim_t always carries tree_t (the reference's config.h:35 defines
  *     ENABLE_TREE); Barnes-Hut versus direct summation is chosen at run time;
  *   - tree_build errors are propagated by sim_solv instead of being ignored
- *     (reference src/simulation.c:414 drops the return code).
+ *     (reference src/simulation.c:312 drops the return code).
  *
  * SOLVER_VIS (reference config.h:32, defined there by default) selects the
  * prototypes of sim_solv, sim_step and solve_gravity_part_tree that also take

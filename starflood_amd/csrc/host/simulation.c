@@ -181,7 +181,7 @@ int sf_solve_all(sf_ctx* c, tree_t* tree, unsigned int step_number, float sph_pr
 }
 
 /* reference src/simulation.c:231-345.  A failed tree build fails the call
-   (the reference ignores tree_build's return code, :414). */
+   (the reference ignores tree_build's return code, :312). */
 int sf_sim_solv(sim_t* SF_RESTRICT sim_ptr) {
 	if (!sim_ptr) return STARFLOOD_FAILURE;
 	sf_ctx* c = sf_ctx_lookup(sim_ptr->mem, "sim_solv");

@@ -222,3 +222,25 @@ def test_entry_points_reject_foreign_sim():
     assert lib.sim_step(C.byref(s)) == L.STARFLOOD_FAILURE
     assert "sim_init" in L.last_error()
     assert lib.solve_gravity_part_part(C.byref(s)) == L.STARFLOOD_FAILURE
+
+
+@pytest.mark.parametrize("solver_vis", [False, True])
+def test_reference_style_c_driver_links_against_the_library(tmp_path, solver_vis):
+    """tests/c/dropin_main.c, a driver making the reference program's calls
+    (src/main.c:358-597), compiles against include/ and links against the
+    library alone, as the reference's Makefile would link it; every
+    reference entry point it calls is resolved from the library."""
+    lib = "starflood_hip_solvervis" if solver_vis else "starflood_hip"
+    exe = tmp_path / "dropin"
+    cmd = ["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", "-Werror", f"-I{REPO / 'include'}",
+           str(REPO / "tests" / "c" / "dropin_main.c"), "-o", str(exe), f"-L{L.LIB_PATH.parent}", f"-l{lib}"]
+    if solver_vis:
+        cmd.insert(1, "-DSOLVER_VIS")
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    out = subprocess.run(["nm", "-D", "--undefined-only", str(exe)], capture_output=True, text=True,
+                         check=True).stdout
+    undefined = {l.split()[-1].split("@")[0] for l in out.splitlines() if l.strip()}
+    calls = {"sim_init", "initcond_generate", "sim_solv", "sim_step", "sim_save_raw", "sim_free",
+             "sf_params_default", "sf_set_params"}
+    assert calls <= undefined
+    assert calls <= exported(L.LIB_PATH.parent / f"lib{lib}.so")

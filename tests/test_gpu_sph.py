@@ -10,7 +10,7 @@ of bit equality."""
 import numpy as np
 import pytest
 
-from conftest import golden, ulp_diff
+from conftest import SPH_GAMMA, SPH_RHO0, golden, sph_lattice, ulp_diff
 from starflood_amd import _lib as L
 from starflood_amd.sim import Simulation
 
@@ -195,6 +195,41 @@ def test_cubic_gas_sphere_conserves_energy_and_momentum():
         assert abs(E1 - E0) <= 1e-4 * abs(E0)
         assert e1["U"] != e0["U"]                      # the gas did work
         assert np.abs(e1["P"]).max() <= 1e-6
+
+
+@pytest.mark.parametrize("search", [L.SF_SPH_SEARCH_GRID, L.SF_SPH_SEARCH_ALL])
+def test_cubic_model_lattice_closed_forms(search):
+    """The closed forms that pin the oracle's pressure force and du/dt, on
+    the device: a 24^3 lattice gas at density 3.7 with support radius 4
+    spacings.  u linear in x gives a = -(gamma - 1) grad u; the homologous
+    flow v = alpha (x - c) gives du/dt = -3 alpha P/rho and no force.  Bound
+    2e-3 on the interior particles (lattice moment error 1.3e-4, FP32 pair
+    sums)."""
+    pos, mas, h, inner = sph_lattice()
+    N = len(mas)
+    gu, alpha = 0.2, -0.3
+    cases = [(0.05 + gu * (pos[:, 0] - 0.5), np.zeros((N, 3))), (np.full(N, 0.05), alpha * (pos - 0.5))]
+    with cubic_sim(N, mem_mode=L.SF_MEM_MIRROR, sph_search=search) as s:
+        for k, (u, vel) in enumerate(cases):
+            s.pos[:] = pos
+            s.vel[:] = vel
+            s.mas[:] = mas
+            s.rad[:] = h
+            s.acc[:] = 0
+            s.set_u(u.astype(np.float32))
+            s.solve_sph()
+            rho = s.rho.astype(np.float64)
+            assert np.allclose(rho[inner], SPH_RHO0, rtol=2e-3)
+            a = s.acc.astype(np.float64)
+            du = s.get_dudt().astype(np.float64)
+            if k == 0:
+                a_exp = -(SPH_GAMMA - 1.0) * gu
+                assert np.abs(a[inner, 0] / a_exp - 1.0).max() <= 2e-3
+                assert np.abs(a[inner, 1:]).max() <= 1e-4 * abs(a_exp)
+            else:
+                du_exp = -3.0 * alpha * s.prs.astype(np.float64) / rho
+                assert np.abs(du[inner] / du_exp[inner] - 1.0).max() <= 2e-3
+                assert np.abs(a[inner]).max() <= 1e-4 * np.abs(du_exp[inner]).max()
 
 
 @pytest.mark.parametrize("N", [1, 2, 37])
