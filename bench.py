@@ -65,58 +65,74 @@ def kernel_ms(lib, name):
     return ms.value, cnt.value
 
 
-def cpu_baseline(budget_s: float = 12.0) -> dict:
-    """The reference's own OpenMP PP solver (a build of its sources in
-    oracle/_ref) timed on this host, on a bounded sample: the same Plummer
-    workload at a smaller N, chosen so the timed work takes ~budget_s."""
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(budget_s: float = 6.0) -> dict:
+    """The reference's own OpenMP PP solver (builds of its sources in
+    oracle/_ref) timed on this host: the Makefile's commented "tuned" flags
+    on the full N = 262144 Plummer state (one evaluation, ~19 s on 16
+    threads), and its default flags on a bounded sample of the same workload
+    (the N whose evaluation takes ~budget_s)."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     kind = "reference" if O.ref_available("pp_e2e-3_dt1e-3") else "port"
-    variant = ["pp_e2e-3_dt1e-3"]
 
-    # probe at a small N to size the sample
-    def run(N, reps):
+    def state(N):
         s = Simulation(N, enable_tree=0, logging=0, epsilon=EPS, dt=DT)
         s.ic_plummer(seed=SEED, first=0, count=N)
         pos, mas = s.pos.copy(), s.mas.copy()
         s.close()
+        return pos, mas
+
+    def run(variant, N, reps):
+        pos, mas = state(N)
         if kind == "reference":
-            r = O.RefSim(variant[0], N)
+            r = O.RefSim(variant, N)
             r.block(0, 3)[:] = pos.ravel()
             r.block(9)[:] = mas
-            r.solve_pp()
             t0 = time.perf_counter()
             for _ in range(reps):
                 r.solve_pp()
             t = (time.perf_counter() - t0) / reps
             r.close()
         else:
-            O.pp_ref(pos, mas, EPS)
             t0 = time.perf_counter()
             for _ in range(reps):
                 O.pp_ref(pos, mas, EPS)
             t = (time.perf_counter() - t0) / reps
         return t
-    t_small = run(8192, 1)
+    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}
+    out = {"unit": "Ginteractions/s", "cores": threads, "kind": kind, "cpu_model": cpu_model(),
+           "logical_cpus_visible": os.cpu_count(), "omp_env": env}
+    tuned = "pp_e2e-3_dt1e-3_tuned"
+    if kind == "reference" and O.ref_available(tuned):
+        run(tuned, 8192, 1)                                   # warm the thread pool
+        t = run(tuned, N_PP, 1)
+        out["value"] = N_PP * float(N_PP) / t / 1e9
+        out["sample"] = (f"solve_gravity_part_part of the reference (-fopenmp -O2 -march=x86-64-v3 -std=c99 build of "
+                         f"its sources, the Makefile's tuned line) on the full headline state, Plummer N={N_PP}, one "
+                         f"evaluation ({t:.1f} s), OMP_NUM_THREADS={threads}")
+        out["full_n_seconds"] = t
+    t_small = run("pp_e2e-3_dt1e-3", 8192, 1)
     rate = 8192.0 ** 2 / t_small
-    N = int(np.sqrt(rate * budget_s / 2.0))
+    N = int(np.sqrt(rate * budget_s))
     N = max(8192, min(N_PP, (N // 4096) * 4096))
-    t = run(N, 2)
-    out = {"value": N * float(N) / t / 1e9, "unit": "Ginteractions/s", "cores": threads, "kind": kind,
-           "sample": f"solve_gravity_part_part on the same Plummer workload at N={N} (2 timed evaluations, "
-                     f"-fopenmp -Og -std=c99 build of the reference sources, OMP_NUM_THREADS={threads})"}
-    if kind == "reference" and O.ref_available("pp_e2e-3_dt1e-3_tuned"):
-        # the reference Makefile's commented "tuned" flags: reported beside
-        # the default build, and the value is the faster of the two
-        variant_default = variant[0]
-        variant[0] = "pp_e2e-3_dt1e-3_tuned"
-        t_tuned = run(N, 2)
-        variant[0] = variant_default
-        out["tuned"] = {"value": N * float(N) / t_tuned / 1e9, "flags": "-fopenmp -O2 -march=x86-64-v3 -std=c99"}
-        if t_tuned < t:
-            out["value"] = out["tuned"]["value"]
-            out["sample"] = out["sample"].replace("-fopenmp -Og -std=c99", "-fopenmp -O2 -march=x86-64-v3 -std=c99")
-            out["default_build_value"] = N * float(N) / t / 1e9
+    t = run("pp_e2e-3_dt1e-3", N, 1)
+    default = {"value": N * float(N) / t / 1e9, "N": N, "flags": "-fopenmp -Og -std=c99 (the reference Makefile's "
+               "default)", "sample": f"the same Plummer workload at N={N}, one evaluation ({t:.1f} s)"}
+    if "value" in out:
+        out["default_build"] = default
+    else:
+        out["value"] = default["value"]
+        out["sample"] = f"solve_gravity_part_part, {default['flags']}, {default['sample']}"
     if O.ref_available(REFTREE_VARIANT):
         # the reference's own tree solve (sim_solv: bounds, octree, walk) on
         # the reference-tree workload
@@ -394,7 +410,7 @@ def bench_bh(steps: int = 3, pp_compare: bool = True) -> dict:
     return out
 
 
-def _kdk_run(N, steps, *, ic, theta=None, leaf=4, eps=EPS, stride=None):
+def _kdk_run(N, steps, *, ic, theta=None, leaf=None, eps=EPS, stride=None, quadrupole=0):
     """steps KDK steps from the IC in the fast policy; FP64 E = T + W before
     and after, and the relative force error of the final state against the
     FP64 direct sum on every stride-th particle."""
@@ -403,7 +419,8 @@ def _kdk_run(N, steps, *, ic, theta=None, leaf=4, eps=EPS, stride=None):
     if theta is None:
         s = Simulation(N, enable_tree=0, **kw)
     else:
-        s = Simulation(N, enable_tree=1, tree_mode=L.SF_TREE_LBVH, theta=theta, leaf_size=leaf, **kw)
+        tree = dict(theta=theta, quadrupole=quadrupole) | ({"leaf_size": leaf} if leaf else {})
+        s = Simulation(N, enable_tree=1, tree_mode=L.SF_TREE_LBVH, **tree, **kw)
     ic(s)
     s.solve()
     e0 = s.stats()
@@ -430,7 +447,8 @@ def bench_bh1m() -> dict:
     theta = 0.5, eps = 0.005, dt = 1e-3, 20 KDK steps; force error against the
     FP64 direct sum on every 256th particle."""
     r = _kdk_run(1048576, 20, ic=lambda s: s.ic_plummer(seed=3, first=0, count=s.N), theta=0.5, eps=5e-3, stride=256)
-    r["workload"] = "BH theta=0.5, Plummer N=1048576, eps=0.005, dt=1e-3, 20 KDK steps, leaf 8"
+    r["workload"] = ("BH (LBVH, monopole, default leaf size) theta=0.5, Plummer N=1048576 (seed 3), eps=0.005, "
+                     "dt=1e-3, 20 KDK steps, fast policy, state resident")
     return r
 
 
@@ -438,7 +456,9 @@ def bench_stress() -> dict:
     """Precision and scale: PP at N = 262144 and BH with theta in {0.3, 0.5,
     0.7} at N = 4194304 (Plummer a = 1, seeded, eps = 0.002), 10 KDK steps
     each: error percentiles against the FP64 direct sum and energy drift."""
-    out = {"workload": "Plummer a=1, eps=0.002, dt=1e-3, 10 KDK steps, fast policy; errors vs FP64 direct sum"}
+    out = {"workload": "Plummer a=1 (seed 5), eps=0.002, dt=1e-3, 10 KDK steps, fast policy, state resident; "
+                       "errors of the final accelerations vs the FP64 direct sum; BH monopole cells at the default "
+                       "leaf size"}
     out["pp_262144"] = _kdk_run(262144, 10, ic=lambda s: s.ic_plummer(seed=5, first=0, count=s.N), stride=64)
     for th in (0.3, 0.5, 0.7):
         out[f"bh_4194304_theta{th}"] = _kdk_run(4194304, 10, ic=lambda s: s.ic_plummer(seed=5, first=0, count=s.N),
@@ -525,6 +545,77 @@ def bench_adjoint(reps: int = 3, ckpt: int = 20) -> dict:
             "grad_norm_v0": float(tv.grad.norm().item())}
 
 
+def _acc64(x, m, eps):
+    """FP64 direct-sum acceleration on the device (torch), dense form:
+    r_ij^2 = |x_i|^2 + |x_j|^2 - 2 x_i.x_j (its FP64 rounding is ~1e-11 of
+    eps^2 at these scales) and sum_j w_ij (x_j - x_i) = W x - x rowsum(W)."""
+    import torch
+    x2 = (x * x).sum(1)
+    s2 = torch.addmm((x2[:, None] + x2[None, :]) + eps * eps, x, x.t().contiguous(), alpha=-2.0)
+    w = m[None, :] * s2.pow(-1.5)
+    return w @ x - x * w.sum(1, keepdim=True)
+
+
+def _kdk64(x, v, m, steps, eps, dt):
+    """A plain FP64 KDK (torch, device): the finite-difference reference."""
+    h = 0.5 * dt
+    a = _acc64(x, m, eps)
+    for _ in range(steps):
+        v = v + h * a
+        x = x + dt * v
+        a = _acc64(x, m, eps)
+        v = v + h * a
+    return x, v
+
+
+def bench_adjoint_fp64(ckpt: int = 20, directions: int = 2) -> dict:
+    """Accuracy of the adjoint workload against FP64: the FP32 force of the
+    initial state against the FP64 direct sum (every particle), and the
+    device gradient (FP32 adjoint, a checkpoint every `ckpt` steps, driven
+    with the FP64 forward's cotangent) against FP64 central differences of
+    the loss along random directions in (x0, v0)."""
+    import torch
+    from starflood_amd import adjoint as A
+    N, K, eps, dt = 16384, 200, 1e-2, 1e-3
+    s = Simulation(N, enable_tree=0, logging=0, epsilon=eps, dt=dt, precision=L.SF_PRECISION_FAST,
+                   acc_mode=L.SF_ACC_PHYSICAL)
+    s.ic_plummer(seed=31, first=0, count=N // 2, a=0.5, M=0.5, centre=(-1.5, 0, 0), bulk_v=(0.15, 0, 0))
+    s.ic_plummer(seed=32, first=N // 2, count=N // 2, a=0.5, M=0.5, centre=(1.5, 0, 0), bulk_v=(-0.15, 0, 0))
+    x0, v0, m = s.pos.copy(), s.vel.copy(), s.mas.copy()
+    ref, _ = s.direct_f64(np.arange(N, dtype=np.uint32))
+    s.close()
+    L.set_params(epsilon=eps, dt=dt, G=1.0, acc_mode=L.SF_ACC_PHYSICAL)
+    dev = torch.device("cuda")
+    acc0, _ = A.force(torch.from_numpy(x0).to(dev), torch.from_numpy(m).to(dev))
+    out = {"workload": f"two Plummer spheres N={N}, eps={eps}, dt={dt}, {K} KDK steps, checkpoint every {ckpt}, "
+                       "MSE loss on final positions",
+           "force_err_vs_fp64_initial_state": rel_err_pcts(acc0.cpu().numpy().astype(np.float64), ref)}
+    d64 = lambda a: torch.tensor(np.asarray(a, np.float64), device=dev)
+    x64, v64, m64 = d64(x0), d64(v0), d64(m)
+    rng = np.random.default_rng(33)
+    with torch.no_grad():
+        xt, _ = _kdk64(d64(x0 + rng.normal(scale=1e-3, size=x0.shape)), v64, m64, K, eps, dt)
+        xN64, _ = _kdk64(x64, v64, m64, K, eps, dt)
+    tx = torch.from_numpy(x0).to(dev).requires_grad_(True)
+    tv = torch.from_numpy(v0).to(dev).requires_grad_(True)
+    xN, _ = A.kdk(tx, tv, torch.from_numpy(m).to(dev), K, ckpt)
+    out["final_position_max_abs_diff_fp32_vs_fp64"] = float((xN.detach().double() - xN64).abs().max().item())
+    xN.backward((2.0 * (xN64 - xt) / xN64.numel()).float())
+    gx, gv = tx.grad.cpu().numpy().astype(np.float64), tv.grad.cpu().numpy().astype(np.float64)
+    hstep, fds = 1e-5, []
+    for _ in range(directions):
+        dx, dv = rng.normal(size=x0.shape), rng.normal(size=v0.shape)
+        with torch.no_grad():
+            lp = ((_kdk64(x64 + hstep * d64(dx), v64 + hstep * d64(dv), m64, K, eps, dt)[0] - xt) ** 2).mean()
+            lm = ((_kdk64(x64 - hstep * d64(dx), v64 - hstep * d64(dv), m64, K, eps, dt)[0] - xt) ** 2).mean()
+        fd = float((lp - lm).item()) / (2.0 * hstep)
+        dd = float((gx * dx).sum() + (gv * dv).sum())
+        fds.append({"adjoint": dd, "central_difference": fd, "rel_diff": abs(dd - fd) / abs(fd)})
+    out["gradient_vs_fp64_central_differences"] = fds
+    L.set_params()
+    return out
+
+
 def bench_sph(steps: int = 50, search: int = L.SF_SPH_SEARCH_ALL) -> dict:
     """PP gravity + cubic-spline SPH: N = 65536 gas particles uniform in the
     unit ball (seeded), v = 0, m = 1/N, gamma = 5/3, u = 0.05, h = 0.05,
@@ -559,6 +650,8 @@ def bench_sph(steps: int = 50, search: int = L.SF_SPH_SEARCH_ALL) -> dict:
     pp_ms, pp_n = kernel_ms(lib, "pp_fast")
     lib.sf_prof_enable(0)
     e1 = s.stats()
+    s.to_host()
+    final = (s.pos.copy(), s.vel.copy(), s.mas.copy(), s.rad.copy(), s.get_u())
     s.close()
     E0 = e0["K"] + e0["W"] + e0["U"]
     E1 = e1["K"] + e1["W"] + e1["U"]
@@ -570,6 +663,46 @@ def bench_sph(steps: int = 50, search: int = L.SF_SPH_SEARCH_ALL) -> dict:
            "momentum_abs": float(np.abs(e1["P"]).max()), "E0": E0, "E1": E1}
     if grid:
         out["sph_grid_sort_ms"] = srt_ms / max(srt_n, 1)
+    out["fp64_error_final_state"] = _sph_fp64_errors(*final, search)
+    return out
+
+
+def _sph_fp64_errors(pos, vel, mas, rad, u, search, stride=64):
+    """Errors of the FP32 solvers on one gas state against the FP64
+    references on the device, every stride-th particle: PP gravity vs the
+    FP64 direct sum (sf_pp_direct_f64), and the cubic SPH density, pressure
+    acceleration and du/dt vs the FP64 all-pairs sums (sf_sph_direct_f64).
+    Relative per particle; acc and du/dt relative to max(|ref_i|, 1e-3
+    max|ref|)."""
+    N = len(mas)
+    rows = np.arange(0, N, stride, dtype=np.uint32)
+    out = {"sample": f"every {stride}th particle ({len(rows)})"}
+    with Simulation(N, enable_tree=0, epsilon=1e-2, dt=5e-4, logging=0, precision=L.SF_PRECISION_FAST,
+                    acc_mode=L.SF_ACC_PHYSICAL, mem_mode=L.SF_MEM_MIRROR) as g:
+        g.pos[:] = pos
+        g.mas[:] = mas
+        g.solve_gravity_pp()
+        ref, _ = g.direct_f64(rows)
+        out["gravity_acc"] = rel_err_pcts(g.acc[rows].astype(np.float64), ref)
+    with Simulation(N, enable_tree=0, enable_sph=1, sph_model=L.SF_SPH_CUBIC_IDEAL, sph_gamma=5.0 / 3.0,
+                    epsilon=1e-2, dt=5e-4, logging=0, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL,
+                    mem_mode=L.SF_MEM_MIRROR, sph_search=search) as h:
+        h.pos[:] = pos
+        h.vel[:] = vel
+        h.mas[:] = mas
+        h.rad[:] = rad
+        h.acc[:] = 0
+        h.set_u(u)
+        h.solve_sph()
+        rho64, acc64, du64 = h.sph_direct_f64(rows)
+        e_rho = np.abs(h.rho[rows] - rho64) / rho64
+        na = np.linalg.norm(acc64, axis=1)
+        e_acc = np.linalg.norm(h.acc[rows].astype(np.float64) - acc64, axis=1) / np.maximum(na, 1e-3 * na.max())
+        du = h.get_dudt()[rows].astype(np.float64)
+        e_du = np.abs(du - du64) / np.maximum(np.abs(du64), 1e-3 * np.abs(du64).max())
+    pct = lambda e: {"p50": float(np.percentile(e, 50)), "p99": float(np.percentile(e, 99)), "max": float(e.max())}
+    out.update(sph_rho=pct(e_rho), sph_acc=pct(e_acc), sph_dudt=pct(e_du))
+    L.set_params()
     return out
 
 
@@ -580,8 +713,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the BH, adjoint and SPH measurements")
-    ap.add_argument("--only", choices=["bh", "adjoint", "sph", "sphgrid", "copy", "bh1m", "stress", "reftree",
-                                       "vis", "run", "mirror"],
+    ap.add_argument("--only", choices=["bh", "adjoint", "adjoint64", "sph", "sphgrid", "copy", "bh1m", "stress",
+                                       "reftree", "vis", "run", "mirror"],
                     help="run one secondary workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -592,7 +725,7 @@ def main():
         fn = {"bh": lambda: bench_bh(steps=args.steps, pp_compare=False), "adjoint": lambda: {"ckpt20": bench_adjoint(reps=1), "ckpt1": bench_adjoint(reps=1, ckpt=1)},
               "sph": lambda: bench_sph(steps=args.steps),
               "sphgrid": lambda: bench_sph(steps=args.steps, search=L.SF_SPH_SEARCH_GRID), "copy": bench_copy,
-              "bh1m": bench_bh1m,
+              "bh1m": bench_bh1m, "adjoint64": bench_adjoint_fp64,
               "stress": bench_stress, "reftree": bench_reftree, "vis": bench_vis, "run": bench_run,
               "mirror": bench_pp_mirror}[args.only]
         print(json.dumps(fn()))
@@ -647,8 +780,9 @@ def main():
         "energy_check": {"K": st["K"], "W": st["W"], "T_ken": st["T"]},
     }
     if not args.no_extra:
-        out["extra"] = {"bh_4m": bench_bh(), "adjoint_16k": bench_adjoint(),
-                        "adjoint_16k_ckpt1": bench_adjoint(ckpt=1), "sph_65k": bench_sph(),
+        out["extra"] = {"bh_4m": bench_bh(), "bh_1m": bench_bh1m(), "stress_4m": bench_stress(),
+                        "adjoint_16k": bench_adjoint(), "adjoint_16k_ckpt1": bench_adjoint(ckpt=1),
+                        "adjoint_16k_fp64": bench_adjoint_fp64(), "sph_65k": bench_sph(),
                         "sph_65k_grid": bench_sph(search=L.SF_SPH_SEARCH_GRID), "reftree_65k": bench_reftree(),
                         "vis_65k": bench_vis(), "run_16k": bench_run(), "pp_262k_mirror": bench_pp_mirror()}
     if not args.no_cpu_baseline:

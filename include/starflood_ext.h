@@ -203,6 +203,12 @@ int sf_prof_query(const char* name, double* total_ms, unsigned long* count);
    solvers, self pair included.  A precision reference, not a solver. */
 int sf_pp_direct_f64(const sim_t* sim_ptr, const unsigned int* idx, unsigned int n_idx,
                      double* acc_out, double* pot_out);
+/* FP64 cubic-model SPH (SF_SPH_CUBIC_IDEAL) over every pair, for the listed
+   particles of the current state (pos, vel, mas, rad, u): density, pressure
+   acceleration (acc_mode semantics) and du/dt into host arrays of n_idx,
+   3 n_idx and n_idx doubles.  A precision reference, not a solver. */
+int sf_sph_direct_f64(const sim_t* sim_ptr, const unsigned int* idx, unsigned int n_idx,
+                      double* rho_out, double* acc_out, double* dudt_out);
 
 /* --- adjoint of the force and of the KDK step (device pointers) ---
    State: x, v are 3N floats (AoS), m is N floats, all device memory.
@@ -216,7 +222,12 @@ typedef struct sf_tape sf_tape;
    a tape for sf_adjoint_backward.  `stream` is the caller's hipStream_t
    (NULL: HIP's default null stream); the work is ordered on it and these
    calls return without waiting for it, except sf_force_dev and
-   sf_force_vjp_dev, which synchronise it. */
+   sf_force_vjp_dev, which synchronise it.  A tape's device memory is
+   allocated (hipMalloc, which may synchronise) when it is made;
+   sf_tape_free keeps the last freed tape's memory, and the next
+   sf_adjoint_forward with the same N, nsteps and checkpoint interval (every
+   iteration of a training loop after the first) reuses it without
+   allocating, once that tape's queued work is done (a device-side wait). */
 int sf_adjoint_forward(unsigned int N, const float* x0, const float* v0, const float* m,
                        unsigned int nsteps, unsigned int ckpt_every,
                        float* xN, float* vN, void* stream, sf_tape** tape_out);

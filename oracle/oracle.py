@@ -77,6 +77,7 @@ def lib() -> C.CDLL:
         L.orc_sim_step.argtypes = [C.POINTER(OrcSim)]
         L.orc_pp_f64.argtypes = [C.c_uint, _f, _f, C.c_double, C.c_double, C.c_int, _u, C.c_uint, _d, _d]
         L.orc_sph_cubic_f64.argtypes = [C.c_uint, _d, _d, _d, _d, _d, C.c_double, _d, _d, _d, _d]
+        L.orc_sph_cubic_rows_f64.argtypes = [C.c_uint, _d, _d, _d, _d, _d, C.c_double, C.c_uint, _u, _d, _d, _d, _d]
         L.orc_cubic_w.argtypes = [C.c_double, C.c_double]
         L.orc_cubic_w.restype = C.c_double
         L.orc_cubic_dw.argtypes = [C.c_double, C.c_double]
@@ -166,6 +167,20 @@ def sph_cubic_f64(pos, vel, mas, h, u, gamma):
     acc = np.zeros(3 * N)
     lib().orc_sph_cubic_f64(N, dp(pos), dp(vel), dp(mas), dp(h), dp(u), gamma, dp(rho), dp(prs), dp(acc), dp(dudt))
     return rho, prs, acc.reshape(N, 3), dudt
+
+
+def sph_cubic_rows_f64(pos, vel, mas, h, u, gamma, rows):
+    """sph_cubic_f64 with the force rows limited to `rows`: rho and P of all
+    particles, acceleration and du/dt of the listed ones."""
+    d = lambda a: np.ascontiguousarray(a, np.float64).ravel()
+    pos, vel, mas, h, u = d(pos), d(vel), d(mas), d(h), d(u)
+    rows = np.ascontiguousarray(rows, np.uint32)
+    N = len(mas)
+    rho, prs = np.zeros(N), np.zeros(N)
+    acc, dudt = np.zeros(3 * len(rows)), np.zeros(len(rows))
+    lib().orc_sph_cubic_rows_f64(N, dp(pos), dp(vel), dp(mas), dp(h), dp(u), gamma, len(rows),
+                                 rows.ctypes.data_as(C.POINTER(C.c_uint32)), dp(rho), dp(prs), dp(acc), dp(dudt))
+    return rho, prs, acc.reshape(-1, 3), dudt
 
 
 def _d64(a):

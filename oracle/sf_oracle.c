@@ -488,8 +488,8 @@ double orc_cubic_dw(double r, double h) {
 	return 0.0;
 }
 
-void orc_sph_cubic_f64(unsigned N, const double* pos, const double* vel, const double* mas, const double* h,
-                       const double* u, double gamma, double* rho, double* prs, double* acc, double* dudt) {
+static void sph_cubic_density(unsigned N, const double* pos, const double* mas, const double* h, const double* u,
+                              double gamma, double* rho, double* prs) {
 #pragma omp parallel for schedule(dynamic, 64)
 	for (unsigned i = 0u; i < N; i++) {
 		double s = 0.0;
@@ -502,28 +502,46 @@ void orc_sph_cubic_f64(unsigned N, const double* pos, const double* vel, const d
 		rho[i] = s;
 		prs[i] = (gamma - 1.0) * s * u[i];
 	}
-#pragma omp parallel for schedule(dynamic, 64)
-	for (unsigned i = 0u; i < N; i++) {
-		double a[3] = {0.0, 0.0, 0.0}, du = 0.0;
-		const double ci = prs[i] / (rho[i] * rho[i]);
-		for (unsigned j = 0u; j < N; j++) {
-			if (j == i) continue;
-			const double d[3] = {pos[3u * i] - pos[3u * j], pos[3u * i + 1u] - pos[3u * j + 1u],
-			                     pos[3u * i + 2u] - pos[3u * j + 2u]};
-			const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-			if (r <= 0.0) continue;
-			const double dw = 0.5 * (orc_cubic_dw(r, h[i]) + orc_cubic_dw(r, h[j]));
-			if (dw == 0.0) continue;
-			const double cj = prs[j] / (rho[j] * rho[j]);
-			const double f = mas[j] * (ci + cj) * dw / r;   /* grad_i Wbar = dw * d / r */
-			const double vd = (vel[3u * i] - vel[3u * j]) * d[0] + (vel[3u * i + 1u] - vel[3u * j + 1u]) * d[1] +
-			                  (vel[3u * i + 2u] - vel[3u * j + 2u]) * d[2];
-			for (int k = 0; k < 3; k++) a[k] -= f * d[k];
-			du += 0.5 * f * vd;
-		}
-		for (int k = 0; k < 3; k++) acc[3u * i + k] = a[k];
-		dudt[i] = du;
+}
+
+/* pressure acceleration and du/dt of particle i, given every rho and P */
+static void sph_cubic_force_row(unsigned N, unsigned i, const double* pos, const double* vel, const double* mas,
+                                const double* h, const double* rho, const double* prs, double a[3], double* dudt) {
+	double du = 0.0;
+	a[0] = a[1] = a[2] = 0.0;
+	const double ci = prs[i] / (rho[i] * rho[i]);
+	for (unsigned j = 0u; j < N; j++) {
+		if (j == i) continue;
+		const double d[3] = {pos[3u * i] - pos[3u * j], pos[3u * i + 1u] - pos[3u * j + 1u],
+		                     pos[3u * i + 2u] - pos[3u * j + 2u]};
+		const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+		if (r <= 0.0) continue;
+		const double dw = 0.5 * (orc_cubic_dw(r, h[i]) + orc_cubic_dw(r, h[j]));
+		if (dw == 0.0) continue;
+		const double cj = prs[j] / (rho[j] * rho[j]);
+		const double f = mas[j] * (ci + cj) * dw / r;   /* grad_i Wbar = dw * d / r */
+		const double vd = (vel[3u * i] - vel[3u * j]) * d[0] + (vel[3u * i + 1u] - vel[3u * j + 1u]) * d[1] +
+		                  (vel[3u * i + 2u] - vel[3u * j + 2u]) * d[2];
+		for (int k = 0; k < 3; k++) a[k] -= f * d[k];
+		du += 0.5 * f * vd;
 	}
+	*dudt = du;
+}
+
+void orc_sph_cubic_f64(unsigned N, const double* pos, const double* vel, const double* mas, const double* h,
+                       const double* u, double gamma, double* rho, double* prs, double* acc, double* dudt) {
+	sph_cubic_density(N, pos, mas, h, u, gamma, rho, prs);
+#pragma omp parallel for schedule(dynamic, 64)
+	for (unsigned i = 0u; i < N; i++) sph_cubic_force_row(N, i, pos, vel, mas, h, rho, prs, acc + 3u * i, dudt + i);
+}
+
+void orc_sph_cubic_rows_f64(unsigned N, const double* pos, const double* vel, const double* mas, const double* h,
+                            const double* u, double gamma, unsigned n_rows, const unsigned* rows, double* rho,
+                            double* prs, double* acc_rows, double* dudt_rows) {
+	sph_cubic_density(N, pos, mas, h, u, gamma, rho, prs);
+#pragma omp parallel for schedule(dynamic, 16)
+	for (unsigned k = 0u; k < n_rows; k++)
+		sph_cubic_force_row(N, rows[k], pos, vel, mas, h, rho, prs, acc_rows + 3u * k, dudt_rows + k);
 }
 
 void orc_force_f64(unsigned N, const double* x, const double* m, double eps, double G, int acc_phys, double* acc) {

@@ -79,6 +79,11 @@ void orc_pp_f64(unsigned N, const float* pos, const float* mas, double eps, doub
    Wbar_ij = (W(r, h_i) + W(r, h_j)) / 2, W the M4 spline of support h. */
 void orc_sph_cubic_f64(unsigned N, const double* pos, const double* vel, const double* mas, const double* h,
                        const double* u, double gamma, double* rho, double* prs, double* acc, double* dudt);
+/* the same for the listed rows only: rho and P of all N (the force rows
+   need their neighbours'), acceleration (3 n_rows) and du/dt (n_rows) */
+void orc_sph_cubic_rows_f64(unsigned N, const double* pos, const double* vel, const double* mas, const double* h,
+                            const double* u, double gamma, unsigned n_rows, const unsigned* rows, double* rho,
+                            double* prs, double* acc_rows, double* dudt_rows);
 double orc_cubic_w(double r, double h);
 double orc_cubic_dw(double r, double h);
 

@@ -122,6 +122,7 @@ _SIG = [
     ("sf_prof_reset", None, []),
     ("sf_prof_query", C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_ulong)]),
     ("sf_pp_direct_f64", C.c_int, [C.POINTER(SimT), C.c_void_p, C.c_uint, C.c_void_p, C.c_void_p]),
+    ("sf_sph_direct_f64", C.c_int, [C.POINTER(SimT), C.c_void_p, C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("sf_adjoint_forward", C.c_int, [C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint, C.c_uint,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     ("sf_adjoint_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

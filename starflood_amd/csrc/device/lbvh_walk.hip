@@ -363,7 +363,10 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
 	if (counters && lane == 0) {
 		atomicAdd(&counters[0], n_cell_tot);
 		atomicAdd(&counters[1], n_part_tot);
-		if (any_bad) atomicAdd(&counters[2], 1ull);
+		if (any_bad) {
+			atomicAdd(&counters[2], 1ull);
+			atomicAdd(&counters[4], 1ull);   // sticky: cleared when the host checks it
+		}
 		atomicAdd(&counters[3], n_batch);
 	}
 }

@@ -15,18 +15,20 @@
 // compiler fetches it with scalar loads into SGPRs.
 //
 // k_pp_fast is the throughput kernel: 13 VALU operations plus one v_rsq_f32
-// per interaction, several i per lane for instruction-level parallelism, j
-// fetched with scalar loads, partial sums per tile of SF_TJ interactions
-// folded into a Kahan-compensated running sum, and the j range split into
-// slices so the grid fills all 256 CUs.  When every particle has the same
-// mass m (k_pack_bodies checks the bits of each mass on the device), the
-// whole tiles of real particles drop the per-pair mass product: they sum
-// 1/s^3 (r_j - r_i) and 1/s with 12 VALU operations plus the rsqrt, and the
-// tile partial is scaled by m before the fold.  Slice s is processed by blocks with
-// blockIdx % slices == s, which the dispatcher places on XCD (s mod 8), so
-// each XCD's L2 holds only its own slice of the body array.  A second kernel
-// combines the slice partials per i in fixed order in FP64, which keeps the
-// result deterministic.
+// per interaction, one i per lane, j fetched with scalar loads, partial sums
+// per tile of SF_TJ interactions folded into a Kahan-compensated running
+// sum, and the j range split into slices so the grid fills all 256 CUs.
+// When every particle has the same mass m (k_pack_bodies checks the bits of
+// each mass on the device), the whole tiles of real particles drop the
+// per-pair mass product: they sum 1/s^3 (r_j - r_i) and 1/s with 12 VALU
+// operations plus the rsqrt, and the tile partial is scaled by m before the
+// fold.  A block's four wavefronts hold the same 64 particles and split its
+// slice into four sub-slices; their sums meet in LDS in FP64, in wavefront
+// order, so a block writes one double4 per particle.  Slice s is processed
+// by blocks with blockIdx % slices == s, which the dispatcher places on XCD
+// (s mod 8), so each XCD's L2 holds only its own part of the body array.
+// A second kernel adds the slice partials per i in fixed order in FP64,
+// which keeps the result deterministic.
 #include <hip/hip_runtime.h>
 
 #include "../sf_internal.h"
@@ -102,16 +104,15 @@ __global__ __launch_bounds__(kBlock) void k_pp_exact(const float* __restrict__ p
 
 // One tile of SF_TJ interactions, j from jt.  nb holds the tile's first
 // kPrefetch bodies on entry and the next tile's first ones on exit (the
-// group after the slice's last is its own first again: in bounds; the
+// group after the range's last is its own first again: in bounds; the
 // compiler then issues the four loads as separate dwordx4 into fresh
 // registers, early in the group -- measured faster than one dwordx16 into
 // the registers being consumed).  UNIFORM: every mass of the tile equals m,
 // and the sums leave it out (sum_j (r_j - r_i)/s^3 and sum_j 1/s).
-template <int IPT, bool WITH_POT, bool UNIFORM>
+template <bool WITH_POT, bool UNIFORM>
 __device__ __forceinline__ void pp_tile(const float4* __restrict__ body, unsigned jt, unsigned j1,
-                                        float4 (&nb)[kPrefetch], const float (&xi)[IPT], const float (&yi)[IPT],
-                                        const float (&zi)[IPT], float eps2, float (&tx)[IPT], float (&ty)[IPT],
-                                        float (&tz)[IPT], float (&tp)[IPT]) {
+                                        float4 (&nb)[kPrefetch], float xi, float yi, float zi, float eps2, float& tx,
+                                        float& ty, float& tz, float& tp) {
 	for (unsigned j = jt; j < jt + SF_TJ; j += kPrefetch) {
 		float4 cb[kPrefetch];
 #pragma unroll
@@ -122,114 +123,109 @@ __device__ __forceinline__ void pp_tile(const float4* __restrict__ body, unsigne
 #pragma unroll
 		for (int u = 0; u < kPrefetch; ++u) {
 			const float4 bj = cb[u];
-#pragma unroll
-			for (int k = 0; k < IPT; ++k) {
-				const float dx = bj.x - xi[k];
-				const float dy = bj.y - yi[k];
-				const float dz = bj.z - zi[k];
-				float r2 = __builtin_fmaf(dx, dx, eps2);
-				r2 = __builtin_fmaf(dy, dy, r2);
-				r2 = __builtin_fmaf(dz, dz, r2);
-				const float inv = __builtin_amdgcn_rsqf(r2);
-				const float mi = UNIFORM ? inv : bj.w * inv;
-				const float inv2 = inv * inv;
-				const float w = mi * inv2;
-				tx[k] = __builtin_fmaf(w, dx, tx[k]);
-				ty[k] = __builtin_fmaf(w, dy, ty[k]);
-				tz[k] = __builtin_fmaf(w, dz, tz[k]);
-				if (WITH_POT) tp[k] += mi;
-			}
+			const float dx = bj.x - xi;
+			const float dy = bj.y - yi;
+			const float dz = bj.z - zi;
+			float r2 = __builtin_fmaf(dx, dx, eps2);
+			r2 = __builtin_fmaf(dy, dy, r2);
+			r2 = __builtin_fmaf(dz, dz, r2);
+			const float inv = __builtin_amdgcn_rsqf(r2);
+			const float mi = UNIFORM ? inv : bj.w * inv;
+			const float inv2 = inv * inv;
+			const float w = mi * inv2;
+			tx = __builtin_fmaf(w, dx, tx);
+			ty = __builtin_fmaf(w, dy, ty);
+			tz = __builtin_fmaf(w, dz, tz);
+			if (WITH_POT) tp += mi;
 		}
 	}
 }
 
-// Accumulates, for IPT particles per lane, sum_j m_j (r_j - r_i) / s^3 and
-// sum_j m_j / s over one j slice.  part holds, per slice, two planes of
-// n_pad float4: the running sum and its Kahan compensation.  The word after
-// the n_pad bodies holds n_uni: the bodies [0, n_uni) are whole tiles of
-// particles of one mass, body[0].w (0 when the masses differ).
-template <int IPT, bool WITH_POT>
+// Block b owns the 64 particles i of i-block b / slices (one per lane, the
+// same in each of its kWavesPP wavefronts) and j slice b % slices, which its
+// wavefronts split into kWavesPP consecutive sub-slices of sub_len bodies.
+// Each wavefront accumulates sum_j m_j (r_j - r_i) / s^3 and sum_j m_j / s
+// over its sub-slice: plain FP32 sums over tiles of SF_TJ interactions,
+// folded into Kahan-compensated running sums.  The wavefronts' compensated
+// sums (s - c) are then added in FP64 in wavefront order through LDS, and
+// the block writes one double4 per particle: part[slice][i].  The word
+// after the n_pad bodies holds n_uni: the bodies [0, n_uni) are whole tiles
+// of particles of one mass, body[0].w (0 when the masses differ).
+constexpr unsigned kWavesPP = kBlock / 64;
+template <bool WITH_POT>
 __global__ __launch_bounds__(kBlock) void k_pp_fast(const float4* __restrict__ body, unsigned n_pad,
-                                                    unsigned slices, unsigned slice_len, float eps2,
-                                                    float4* __restrict__ part) {
+                                                    unsigned slices, unsigned sub_len, float eps2,
+                                                    double4* __restrict__ part) {
+	__shared__ double red[kWavesPP][4][64];
 	const unsigned b = blockIdx.x;
 	const unsigned s = b % slices;
-	const unsigned ib = b / slices;
-	const unsigned ibase = ib * (kBlock * IPT) + threadIdx.x;
-
-	float xi[IPT], yi[IPT], zi[IPT];
-	float Sx[IPT], Sy[IPT], Sz[IPT], Sp[IPT];
-	float Cx[IPT], Cy[IPT], Cz[IPT], Cp[IPT];
-#pragma unroll
-	for (int k = 0; k < IPT; ++k) {
-		const float4 bi = body[ibase + k * kBlock];
-		xi[k] = bi.x;
-		yi[k] = bi.y;
-		zi[k] = bi.z;
-		Sx[k] = Sy[k] = Sz[k] = Sp[k] = 0.0f;
-		Cx[k] = Cy[k] = Cz[k] = Cp[k] = 0.0f;
-	}
+	// the wavefront index is uniform per wavefront: readfirstlane tells the
+	// compiler, so the j stream stays in scalar loads and SGPRs
+	const unsigned lane = threadIdx.x & 63u, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const unsigned i = (b / slices) * 64u + lane;
+	const float4 bi = body[i];
+	const float xi = bi.x, yi = bi.y, zi = bi.z;
+	float Sx = 0.0f, Sy = 0.0f, Sz = 0.0f, Sp = 0.0f;
+	float Cx = 0.0f, Cy = 0.0f, Cz = 0.0f, Cp = 0.0f;
 
 	// The j stream keeps one group of kPrefetch bodies in flight: the scalar
 	// loads of the next group are issued before the current group's
 	// arithmetic, so their latency overlaps it.  The equal-mass tiles come
 	// first in a loop of their own: one loop with a branch per tile made the
 	// compiler carry the j stream in VGPRs (measured 8 % slower).
-	const unsigned j0 = s * slice_len;
-	const unsigned j1 = min(j0 + slice_len, n_pad);
-	const unsigned n_uni = *reinterpret_cast<const unsigned*>(body + n_pad);
-	const unsigned ju = min(j1, max(j0, n_uni));
-	const float mu = body[0].w;
-	float4 nb[kPrefetch];
+	const unsigned j0 = min((s * kWavesPP + wv) * sub_len, n_pad);
+	const unsigned j1 = min(j0 + sub_len, n_pad);
+	if (j0 < j1) {
+		const unsigned n_uni = *reinterpret_cast<const unsigned*>(body + n_pad);
+		const unsigned ju = min(j1, max(j0, n_uni));
+		const float mu = body[0].w;
+		float4 nb[kPrefetch];
 #pragma unroll
-	for (int u = 0; u < kPrefetch; ++u) nb[u] = body[j0 + u];
-	unsigned jt = j0;
-	for (; jt < ju; jt += SF_TJ) {
-		float tx[IPT], ty[IPT], tz[IPT], tp[IPT];
-#pragma unroll
-		for (int k = 0; k < IPT; ++k) tx[k] = ty[k] = tz[k] = tp[k] = 0.0f;
-		pp_tile<IPT, WITH_POT, true>(body, jt, j1, nb, xi, yi, zi, eps2, tx, ty, tz, tp);
-#pragma unroll
-		for (int k = 0; k < IPT; ++k) {
-			kahan_add(Sx[k], Cx[k], mu * tx[k]);
-			kahan_add(Sy[k], Cy[k], mu * ty[k]);
-			kahan_add(Sz[k], Cz[k], mu * tz[k]);
-			if (WITH_POT) kahan_add(Sp[k], Cp[k], mu * tp[k]);
+		for (int u = 0; u < kPrefetch; ++u) nb[u] = body[j0 + u];
+		unsigned jt = j0;
+		for (; jt < ju; jt += SF_TJ) {
+			float tx = 0.0f, ty = 0.0f, tz = 0.0f, tp = 0.0f;
+			pp_tile<WITH_POT, true>(body, jt, j1, nb, xi, yi, zi, eps2, tx, ty, tz, tp);
+			kahan_add(Sx, Cx, mu * tx);
+			kahan_add(Sy, Cy, mu * ty);
+			kahan_add(Sz, Cz, mu * tz);
+			if (WITH_POT) kahan_add(Sp, Cp, mu * tp);
+		}
+		for (; jt < j1; jt += SF_TJ) {
+			float tx = 0.0f, ty = 0.0f, tz = 0.0f, tp = 0.0f;
+			pp_tile<WITH_POT, false>(body, jt, j1, nb, xi, yi, zi, eps2, tx, ty, tz, tp);
+			kahan_add(Sx, Cx, tx);
+			kahan_add(Sy, Cy, ty);
+			kahan_add(Sz, Cz, tz);
+			if (WITH_POT) kahan_add(Sp, Cp, tp);
 		}
 	}
-	for (; jt < j1; jt += SF_TJ) {
-		float tx[IPT], ty[IPT], tz[IPT], tp[IPT];
+	// Kahan's compensation c holds the negated lost low part: each sum is s - c
+	red[wv][0][lane] = (double)Sx - (double)Cx;
+	red[wv][1][lane] = (double)Sy - (double)Cy;
+	red[wv][2][lane] = (double)Sz - (double)Cz;
+	red[wv][3][lane] = (double)Sp - (double)Cp;
+	__syncthreads();
+	if (wv != 0) return;
+	double4 t = make_double4(red[0][0][lane], red[0][1][lane], red[0][2][lane], red[0][3][lane]);
 #pragma unroll
-		for (int k = 0; k < IPT; ++k) tx[k] = ty[k] = tz[k] = tp[k] = 0.0f;
-		pp_tile<IPT, WITH_POT, false>(body, jt, j1, nb, xi, yi, zi, eps2, tx, ty, tz, tp);
-#pragma unroll
-		for (int k = 0; k < IPT; ++k) {
-			kahan_add(Sx[k], Cx[k], tx[k]);
-			kahan_add(Sy[k], Cy[k], ty[k]);
-			kahan_add(Sz[k], Cz[k], tz[k]);
-			if (WITH_POT) kahan_add(Sp[k], Cp[k], tp[k]);
-		}
+	for (unsigned q = 1; q < kWavesPP; ++q) {
+		t.x += red[q][0][lane];
+		t.y += red[q][1][lane];
+		t.z += red[q][2][lane];
+		t.w += red[q][3][lane];
 	}
-
-	float4* sum_plane = part + (size_t)(2u * s) * n_pad;
-	float4* com_plane = part + (size_t)(2u * s + 1u) * n_pad;
-#pragma unroll
-	for (int k = 0; k < IPT; ++k) {
-		const unsigned i = ibase + k * kBlock;
-		sum_plane[i] = make_float4(Sx[k], Sy[k], Sz[k], Sp[k]);
-		com_plane[i] = make_float4(Cx[k], Cy[k], Cz[k], Cp[k]);
-	}
+	part[(size_t)s * n_pad + i] = t;
 }
 
-// Fixed-order FP64 combine of the slice partials; Kahan's compensation c
-// holds the negated lost low part, so each slice contributes (s - c).  A
-// block takes kCombI particles; its wavefronts take the slices s = g, g +
-// kCombG, ... (g = wavefront), each lane one particle (coalesced loads), and
-// wavefront 0 adds the kCombG sums in order.  One lane per particle over all
-// slices left most CUs idle at small N (64 blocks at N = 16384: 20 us).
+// Fixed-order FP64 sum of the slice partials.  A block takes kCombI
+// particles; its wavefronts take the slices s = g, g + kCombG, ... (g =
+// wavefront), each lane one particle (coalesced loads), and wavefront 0
+// adds the kCombG sums in order.  One lane per particle over all slices
+// left most CUs idle at small N (64 blocks at N = 16384: 20 us).
 constexpr unsigned kCombI = 64;
 constexpr unsigned kCombG = kBlock / kCombI;
-__global__ __launch_bounds__(kBlock) void k_pp_combine(const float4* __restrict__ part, unsigned n_pad,
+__global__ __launch_bounds__(kBlock) void k_pp_combine(const double4* __restrict__ part, unsigned n_pad,
                                                        unsigned slices, const float4* __restrict__ body,
                                                        unsigned N, double G, int acc_phys,
                                                        float* __restrict__ acc, float* __restrict__ pot) {
@@ -238,12 +234,11 @@ __global__ __launch_bounds__(kBlock) void k_pp_combine(const float4* __restrict_
 	const unsigned i = blockIdx.x * kCombI + il;   // < n_pad: the grid covers N in kCombI steps
 	double sx = 0.0, sy = 0.0, sz = 0.0, sp = 0.0;
 	for (unsigned s = g; s < slices; s += kCombG) {
-		const float4 a = part[(size_t)(2u * s) * n_pad + i];
-		const float4 c = part[(size_t)(2u * s + 1u) * n_pad + i];
-		sx += (double)a.x - (double)c.x;
-		sy += (double)a.y - (double)c.y;
-		sz += (double)a.z - (double)c.z;
-		sp += (double)a.w - (double)c.w;
+		const double4 a = part[(size_t)s * n_pad + i];
+		sx += a.x;
+		sy += a.y;
+		sz += a.z;
+		sp += a.w;
 	}
 	red[g][0][il] = sx;
 	red[g][1][il] = sy;
@@ -344,11 +339,6 @@ __global__ __launch_bounds__(kBlock) void k_pp_f64(const float* __restrict__ pos
 	}
 }
 
-int env_uint(const char* name, int fallback) {
-	const char* v = getenv(name);
-	if (!v || !*v) return fallback;
-	return atoi(v);
-}
 
 }  // namespace
 
@@ -384,45 +374,32 @@ extern "C" int sfd_pp_exact(const float* pos, const float* mas, unsigned N, floa
 extern "C" int sfd_pp_fast(const float* body, unsigned N, unsigned n_pad, float eps2, double G, int acc_phys,
                            float* part, unsigned slices, float* acc, float* pot, hipStream_t s) {
 	if (N == 0) return 0;
-	static int ipt = -1;
-	if (ipt < 0) {
-		ipt = env_uint("SF_PP_IPT", 1);
-		if (ipt != 1 && ipt != 2 && ipt != 4 && ipt != 8) ipt = 1;
-	}
-	const unsigned per_block = kBlock * (unsigned)ipt;
-	if (n_pad % per_block != 0 || n_pad % SF_TJ != 0) {
-		sf_error("sfd_pp_fast", "n_pad %u is not a multiple of %u", n_pad, per_block);
+	if (n_pad % 64u != 0 || n_pad % SF_TJ != 0) {
+		sf_error("sfd_pp_fast", "n_pad %u is not a multiple of %u", n_pad, SF_TJ);
 		return -1;
 	}
-	// slice length: whole SF_TJ tiles, every slice non-empty
-	unsigned tiles = n_pad / SF_TJ;
+	// `slices` j ranges (whole SF_TJ tiles, every one non-empty), one per
+	// wavefront of a block: slices / kWavesPP block slices, each writing one
+	// double4 plane, so part needs ceil(slices / kWavesPP) * n_pad * 32 bytes
+	// (no more than the slices x 2 float4 planes callers reserve)
+	const unsigned tiles = n_pad / SF_TJ;
 	if (slices > tiles) slices = tiles;
-	const unsigned tiles_per_slice = (tiles + slices - 1) / slices;
-	const unsigned slice_len = tiles_per_slice * SF_TJ;
-	slices = (n_pad + slice_len - 1) / slice_len;
-	const unsigned grid = (n_pad / per_block) * slices;
-	float4* p4 = reinterpret_cast<float4*>(part);
+	if (slices < 1u) slices = 1u;
+	const unsigned sub_len = (tiles + slices - 1) / slices * SF_TJ;
+	const unsigned subs = (n_pad + sub_len - 1) / sub_len;
+	const unsigned bslices = (subs + kWavesPP - 1) / kWavesPP;
+	const unsigned grid = (n_pad / 64u) * bslices;
+	double4* p4 = reinterpret_cast<double4*>(part);
 	const float4* b4 = reinterpret_cast<const float4*>(body);
 	const int tok = sf_prof_begin("pp_fast", s);
-	if (pot) {
-		switch (ipt) {
-		case 1: hipLaunchKernelGGL((k_pp_fast<1, true>), dim3(grid), dim3(kBlock), 0, s, b4, n_pad, slices, slice_len, eps2, p4); break;
-		case 2: hipLaunchKernelGGL((k_pp_fast<2, true>), dim3(grid), dim3(kBlock), 0, s, b4, n_pad, slices, slice_len, eps2, p4); break;
-		case 8: hipLaunchKernelGGL((k_pp_fast<8, true>), dim3(grid), dim3(kBlock), 0, s, b4, n_pad, slices, slice_len, eps2, p4); break;
-		default: hipLaunchKernelGGL((k_pp_fast<4, true>), dim3(grid), dim3(kBlock), 0, s, b4, n_pad, slices, slice_len, eps2, p4); break;
-		}
-	} else {
-		switch (ipt) {
-		case 1: hipLaunchKernelGGL((k_pp_fast<1, false>), dim3(grid), dim3(kBlock), 0, s, b4, n_pad, slices, slice_len, eps2, p4); break;
-		case 2: hipLaunchKernelGGL((k_pp_fast<2, false>), dim3(grid), dim3(kBlock), 0, s, b4, n_pad, slices, slice_len, eps2, p4); break;
-		case 8: hipLaunchKernelGGL((k_pp_fast<8, false>), dim3(grid), dim3(kBlock), 0, s, b4, n_pad, slices, slice_len, eps2, p4); break;
-		default: hipLaunchKernelGGL((k_pp_fast<4, false>), dim3(grid), dim3(kBlock), 0, s, b4, n_pad, slices, slice_len, eps2, p4); break;
-		}
-	}
+	if (pot)
+		hipLaunchKernelGGL(k_pp_fast<true>, dim3(grid), dim3(kBlock), 0, s, b4, n_pad, bslices, sub_len, eps2, p4);
+	else
+		hipLaunchKernelGGL(k_pp_fast<false>, dim3(grid), dim3(kBlock), 0, s, b4, n_pad, bslices, sub_len, eps2, p4);
 	sf_prof_end(tok, s);
 	if (sf_hip_check(hipGetLastError(), "sfd_pp_fast", "k_pp_fast launch")) return -1;
 	const int tok2 = sf_prof_begin("pp_combine", s);
-	hipLaunchKernelGGL(k_pp_combine, dim3((N + kCombI - 1) / kCombI), dim3(kBlock), 0, s, p4, n_pad, slices, b4, N,
+	hipLaunchKernelGGL(k_pp_combine, dim3((N + kCombI - 1) / kCombI), dim3(kBlock), 0, s, p4, n_pad, bslices, b4, N,
 	                   G, acc_phys, acc, pot);
 	sf_prof_end(tok2, s);
 	return sf_hip_check(hipGetLastError(), "sfd_pp_fast", "k_pp_combine launch");

@@ -96,8 +96,10 @@ int sfd_lbvh_topology(const float* pos, const float* mas, const unsigned long lo
                       void* tile_sum, void* node_cm, void* node_cell, hipStream_t s);
 int sfd_lbvh_records(unsigned int N, const void* node_lr, const void* node_range, const void* node_cm,
                      const void* node_cell, float theta, int leaf_size, void* rec_b, hipStream_t s);
-/* lbvh_walk.hip.  counters (4 x u64, zeroed here): cell interactions,
-   particle interactions, wavefronts that met a malformed tree, node batches.
+/* lbvh_walk.hip.  counters (5 x u64; the first four zeroed here): cell
+   interactions, particle interactions, wavefronts that met a malformed tree,
+   node batches; the fifth adds the third up over walks until the host clears
+   it.
    gstack: sfd_lbvh_walk_stack_bytes() of scratch. */
 size_t sfd_lbvh_walk_stack_bytes(void);
 int sfd_lbvh_walk(const void* body, const unsigned int* order, unsigned int N, const unsigned int* group_start,

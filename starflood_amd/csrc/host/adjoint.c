@@ -94,16 +94,36 @@ struct sf_tape {
 	unsigned int seg_first;     /* step of seg[0] */
 	float *x, *v, *a, *lx, *lv; /* 3N each */
 	double* x64;                /* 3N */
+	hipStream_t stream;         /* the stream of the tape's last call */
+	hipEvent_t done;            /* recorded there by sf_tape_free */
+	int have_done;
 };
 
-void sf_tape_free(sf_tape* t) {
-	if (!t) return;
-	hipDeviceSynchronize();
+/* Device memory of a tape, released (hipFree waits for the device). */
+static void tape_release(sf_tape* t) {
 	void* ptrs[] = {t->m, t->ck_x64, t->ck_v, t->ck_a, t->seg, t->x, t->v, t->a, t->lx, t->lv, t->x64};
 	for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k)
 		if (ptrs[k]) hipFree(ptrs[k]);
 	scratch_free(&t->w);
+	if (t->have_done) hipEventDestroy(t->done);
 	free(t);
+}
+
+/* The last freed tape, kept with its device memory for the next forward of
+   the same N, step count and checkpoint interval (a training loop's every
+   iteration after the first): no allocation, and no host synchronisation,
+   only a device-side wait on the event recorded when it was freed. */
+static sf_tape* g_spare;
+
+void sf_tape_free(sf_tape* t) {
+	if (!t) return;
+	if (g_spare) tape_release(g_spare);
+	g_spare = NULL;
+	if (t->have_done && hipEventRecord(t->done, t->stream) == hipSuccess) {
+		g_spare = t;
+		return;
+	}
+	tape_release(t);
 }
 
 /* the caller's stream; NULL is HIP's default (null) stream, which is also
@@ -119,6 +139,9 @@ static int kdk(sf_tape* t, hipStream_t s) {
 	return sfd_kick_fast(t->v, t->a, n3, h, s);
 }
 
+static int forward_run(sf_tape* t, const float* x0, const float* v0, const float* m, float* xN, float* vN,
+                       hipStream_t s, const char* fn);
+
 int sf_adjoint_forward(unsigned int N, const float* x0, const float* v0, const float* m, unsigned int nsteps,
                        unsigned int ckpt_every, float* xN, float* vN, void* stream, sf_tape** tape_out) {
 	const char* fn = "sf_adjoint_forward";
@@ -128,15 +151,33 @@ int sf_adjoint_forward(unsigned int N, const float* x0, const float* v0, const f
 		return STARFLOOD_FAILURE;
 	}
 	hipStream_t s = stream_of(stream);
-	sf_tape* t = (sf_tape*)calloc(1, sizeof(sf_tape));
+	const unsigned int ckpt = ckpt_every < nsteps ? ckpt_every : (nsteps ? nsteps : 1u);
+	const size_t n3 = 3u * (size_t)N, f3 = n3 * sizeof(float);
+	sf_tape* t = g_spare;
+	if (t && t->N == N && t->nsteps == nsteps && t->ckpt == ckpt) {
+		/* the spare tape's buffers, once its last call's work is done */
+		g_spare = NULL;
+		int rc = hipStreamWaitEvent(s, t->done, 0) != hipSuccess;
+		t->p = *sf_params();
+		t->stream = s;
+		if (!rc) rc = forward_run(t, x0, v0, m, xN, vN, s, fn);
+		if (rc) {
+			tape_release(t);
+			return STARFLOOD_FAILURE;
+		}
+		*tape_out = t;
+		return STARFLOOD_SUCCESS;
+	}
+	t = (sf_tape*)calloc(1, sizeof(sf_tape));
 	if (!t) return STARFLOOD_FAILURE;
 	t->p = *sf_params();
+	t->stream = s;
 	t->N = N;
 	t->nsteps = nsteps;
-	t->ckpt = ckpt_every < nsteps ? ckpt_every : (nsteps ? nsteps : 1u);
+	t->ckpt = ckpt;
 	t->n_ckpt = nsteps ? (nsteps + t->ckpt - 1u) / t->ckpt : 0u;
 	t->last_start = t->n_ckpt ? (t->n_ckpt - 1u) * t->ckpt : 0u;
-	const size_t n3 = 3u * (size_t)N, f3 = n3 * sizeof(float);
+	t->have_done = hipEventCreateWithFlags(&t->done, hipEventDisableTiming) == hipSuccess;
 	int rc = scratch_alloc(&t->w, N, fn);
 	rc |= hipMalloc((void**)&t->m, (size_t)N * sizeof(float)) != hipSuccess;
 	rc |= hipMalloc((void**)&t->ck_x64, (t->n_ckpt ? t->n_ckpt : 1u) * n3 * sizeof(double)) != hipSuccess;
@@ -151,9 +192,23 @@ int sf_adjoint_forward(unsigned int N, const float* x0, const float* v0, const f
 	rc |= hipMalloc((void**)&t->x64, n3 * sizeof(double)) != hipSuccess;
 	if (rc) {
 		sf_error(fn, "device allocation for N = %u, %u steps failed", N, nsteps);
-		sf_tape_free(t);
+		tape_release(t);
 		return STARFLOOD_FAILURE;
 	}
+	if (forward_run(t, x0, v0, m, xN, vN, s, fn)) {
+		tape_release(t);
+		return STARFLOOD_FAILURE;
+	}
+	*tape_out = t;
+	return STARFLOOD_SUCCESS;
+}
+
+/* The forward integration into tape t (buffers in place) on stream s. */
+static int forward_run(sf_tape* t, const float* x0, const float* v0, const float* m, float* xN, float* vN,
+                       hipStream_t s, const char* fn) {
+	const unsigned int N = t->N, nsteps = t->nsteps;
+	const size_t n3 = 3u * (size_t)N, f3 = n3 * sizeof(float);
+	int rc = 0;
 	rc |= hipMemcpyAsync(t->m, m, (size_t)N * sizeof(float), hipMemcpyDeviceToDevice, s) != hipSuccess;
 	rc |= hipMemcpyAsync(t->x, x0, f3, hipMemcpyDeviceToDevice, s) != hipSuccess;
 	rc |= hipMemcpyAsync(t->v, v0, f3, hipMemcpyDeviceToDevice, s) != hipSuccess;
@@ -181,13 +236,10 @@ int sf_adjoint_forward(unsigned int N, const float* x0, const float* v0, const f
 	rc |= hipMemcpyAsync(xN, t->x, f3, hipMemcpyDeviceToDevice, s) != hipSuccess;
 	rc |= hipMemcpyAsync(vN, t->v, f3, hipMemcpyDeviceToDevice, s) != hipSuccess;
 	if (rc || sf_hip_check(hipGetLastError(), fn, "launch")) {
-		if (!rc) rc = -1;
 		sf_error(fn, "forward integration failed");
-		sf_tape_free(t);
-		return STARFLOOD_FAILURE;
+		return -1;
 	}
-	*tape_out = t;
-	return STARFLOOD_SUCCESS;
+	return 0;
 }
 
 /* The positions of segment c that the backward reads, into t->seg:
@@ -221,6 +273,7 @@ int sf_adjoint_backward(sf_tape* t, const float* gxN, const float* gvN, float* g
 		return STARFLOOD_FAILURE;
 	}
 	hipStream_t s = stream_of(stream);
+	t->stream = s;
 	const unsigned int n3 = 3u * t->N;
 	const size_t f3 = (size_t)n3 * sizeof(float);
 	const float dt = (float)t->p.dt, h = 0.5f * dt;

@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include "../sf_internal.h"
+#include "sf_host.h"
 
 #define SF_MAX_CTX 16
 
@@ -152,7 +153,12 @@ int sf_sync(const sim_t* sim_ptr, int direction) {
 			return STARFLOOD_FAILURE;
 		}
 	}
-	return sf_finish(c, "sf_sync") ? STARFLOOD_FAILURE : STARFLOOD_SUCCESS;
+	int rc = sf_finish(c, "sf_sync");
+	/* resident-mode results reach the host here: report a tree walk cut short */
+	if (!rc && direction == SF_SYNC_TO_HOST && sf_params()->enable_tree && c->N &&
+	    sf_tree_check((tree_t*)&sim_ptr->tree, c->stream, "sf_sync"))
+		rc = -1;
+	return rc ? STARFLOOD_FAILURE : STARFLOOD_SUCCESS;
 }
 
 void* sf_device_find(const sim_t* sim_ptr, enum sim_conf parameter_enum) {

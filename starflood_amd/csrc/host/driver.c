@@ -113,6 +113,14 @@ int sf_run(const sf_run_config_t* cfg) {
 		sf_error(fn, "enable_vis needs vis_sizex and vis_sizey");
 		return STARFLOOD_FAILURE;
 	}
+	/* the cubic model's u lives on the device only: the run loop has no
+	   initial value for it and raw snapshots do not carry it */
+	if (p.enable_sph && p.sph_model == SF_SPH_CUBIC_IDEAL) {
+		sf_error(fn, "the run loop does not carry the cubic SPH model's internal energy u (not in raw "
+		             "snapshots); use sph_model SF_SPH_REFERENCE, or drive sim_step directly with "
+		             "sf_ext_copy(SF_EXT_U)");
+		return STARFLOOD_FAILURE;
+	}
 
 	/* src/main.c:348-412: visualisation, simulation, initial conditions */
 	vis_t vis;

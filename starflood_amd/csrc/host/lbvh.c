@@ -31,7 +31,7 @@ struct sf_lbvh {
 	void* node_cm;            /* 2N-1 float4: com, mass */
 	void* node_cell;          /* 2N-1 float4: Morton cell centre, side */
 	void* rec_b;              /* 2N-1 int4 */
-	unsigned long long* counters;   /* 4 */
+	unsigned long long* counters;   /* 5: the walk's four (zeroed per walk), then the sticky guard count */
 	unsigned long long* h_counters; /* pinned */
 	int* gstack;                    /* walk node stacks, one per wavefront slot */
 	unsigned int* group_start;      /* walk group starts in sorted order */
@@ -61,7 +61,11 @@ void sf_lbvh_destroy(sf_lbvh* t) {
 	free(t);
 }
 
-static int alloc(void** p, size_t bytes) { return hipMalloc(p, bytes ? bytes : 16u) == hipSuccess ? 0 : -1; }
+static int alloc(void** p, size_t bytes) {
+	if (hipMalloc(p, bytes ? bytes : 16u) == hipSuccess) return 0;
+	*p = NULL;
+	return -1;
+}
 
 static int ensure(sf_lbvh** st, unsigned int N) {
 	if (!*st) {
@@ -91,10 +95,12 @@ static int ensure(sf_lbvh** st, unsigned int N) {
 	rc |= alloc(&t->node_cm, nn * 16u);
 	rc |= alloc(&t->node_cell, nn * 16u);
 	rc |= alloc(&t->rec_b, nn * 16u);
-	rc |= alloc((void**)&t->counters, 4u * 8u);
+	rc |= alloc((void**)&t->counters, 5u * 8u);
 	rc |= alloc((void**)&t->gstack, sfd_lbvh_walk_stack_bytes());
 	rc |= alloc((void**)&t->group_start, (size_t)sfd_lbvh_group_flag_len(N) * 4u);
-	if (!rc && hipHostMalloc((void**)&t->h_counters, 4u * 8u, hipHostMallocDefault) != hipSuccess) rc = -1;
+	if (!rc && hipHostMalloc((void**)&t->h_counters, 5u * 8u, hipHostMallocDefault) != hipSuccess) rc = -1;
+	/* the sticky guard count starts at zero (the walk resets only the first four) */
+	if (!rc && (hipMemset(t->counters, 0, 5u * 8u) != hipSuccess || hipDeviceSynchronize() != hipSuccess)) rc = -1;
 	if (rc) {
 		sf_error("tree_build", "device allocation for an LBVH of %u particles failed", N);
 		free_buffers(t);
@@ -105,8 +111,19 @@ static int ensure(sf_lbvh** st, unsigned int N) {
 	return 0;
 }
 
+static void free_quad(sf_lbvh* t) {
+	void** ptrs[] = {&t->prefix2, &t->tile_sum2, (void**)&t->quad_ref, &t->node_q0, (void**)&t->node_q1};
+	for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k) {
+		if (*ptrs[k]) hipFree(*ptrs[k]);
+		*ptrs[k] = NULL;
+	}
+}
+
+/* The quadrupole buffers exist all together or not at all: a partial
+   allocation is released, so the next build tries again from scratch. */
 static int ensure_quad(sf_lbvh* t) {
-	if (t->node_q0) return 0;
+	if (t->prefix2 && t->tile_sum2 && t->quad_ref && t->node_q0 && t->node_q1) return 0;
+	free_quad(t);
 	const size_t N = t->n_alloc, nn = 2u * N;
 	int rc = 0;
 	rc |= alloc(&t->prefix2, N * 48u);
@@ -114,8 +131,12 @@ static int ensure_quad(sf_lbvh* t) {
 	rc |= alloc((void**)&t->quad_ref, 3u * 8u);
 	rc |= alloc(&t->node_q0, nn * 16u);
 	rc |= alloc((void**)&t->node_q1, nn * 4u);
-	if (rc) sf_error("tree_build", "device allocation for LBVH quadrupoles of %zu particles failed", N);
-	return rc ? -1 : 0;
+	if (rc) {
+		sf_error("tree_build", "device allocation for LBVH quadrupoles of %zu particles failed", N);
+		free_quad(t);
+		return -1;
+	}
+	return 0;
 }
 
 int sf_lbvh_build(sf_lbvh** st, unsigned int N, const float* pos, const float* mas, int quadrupole, hipStream_t s) {
@@ -170,6 +191,19 @@ int sf_lbvh_walk(sf_lbvh* t, unsigned int N, const float* pos, const float* mas,
 }
 
 unsigned int sf_lbvh_size(const sf_lbvh* t) { return t ? t->N : 0u; }
+
+/* Wavefronts of the walks since the last call that met a malformed tree or
+   ran out of stack or node budget (their forces are incomplete); returns
+   the count and clears it.  Synchronises s. */
+int sf_lbvh_guard(sf_lbvh* t, hipStream_t s, unsigned long long* tripped) {
+	*tripped = 0;
+	if (!t || !t->counters) return 0;   /* nothing was walked */
+	SF_HIP(hipMemcpyAsync(t->h_counters + 4, t->counters + 4, 8u, hipMemcpyDeviceToHost, s), "sf_lbvh_guard");
+	SF_HIP(hipStreamSynchronize(s), "sf_lbvh_guard");
+	*tripped = t->h_counters[4];
+	if (*tripped) SF_HIP(hipMemsetAsync(t->counters + 4, 0, 8u, s), "sf_lbvh_guard");
+	return 0;
+}
 
 int sf_lbvh_export_impl(sf_lbvh* t, hipStream_t s, unsigned long long* keys, unsigned int* order,
                         unsigned long long counts[4], float* node_cm, int* node_lr_range) {

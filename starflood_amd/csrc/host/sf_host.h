@@ -21,7 +21,7 @@ int sf_pp_force_arrays(const sf_params_t* p, unsigned int N, unsigned int n_pad,
 int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number);
 /* tree.c: after an LBVH solve on stream s, fail if the walk met a malformed
    tree (synchronises s; no-op in reference mode) */
-int sf_tree_check(tree_t* tree, hipStream_t s);
+int sf_tree_check(tree_t* tree, hipStream_t s, const char* fn);
 /* sph.c: density, EOS and pressure force, added into acc.  For the cubic
    model sf_sph_prepare packs the particle records first, while acc still
    holds the previous forces: velocities and u are predicted pred_dt ahead. */
@@ -43,6 +43,9 @@ int sf_lbvh_walk(sf_lbvh* st, unsigned int N, const float* pos, const float* mas
 void sf_lbvh_destroy(sf_lbvh* st);
 /* particles of the LBVH last built (0: none, or no build yet) */
 unsigned int sf_lbvh_size(const sf_lbvh* st);
+/* walk wavefronts that met a malformed tree since the last call (cleared;
+   synchronises s) */
+int sf_lbvh_guard(sf_lbvh* st, hipStream_t s, unsigned long long* tripped);
 /* synchronises s, then copies out what is asked for (NULL: skip) */
 int sf_lbvh_export_impl(sf_lbvh* st, hipStream_t s, unsigned long long* keys, unsigned int* order,
                         unsigned long long counts[4], float* node_cm, int* node_lr_range);

@@ -191,7 +191,7 @@ int sf_sim_solv(sim_t* SF_RESTRICT sim_ptr) {
 	if (sf_solve_all(c, &sim_ptr->tree, sim_ptr->step_number, 0.0f)) rc = -1;
 	if (download_solver_outputs(c)) rc = -1;
 	if (sf_finish(c, "sim_solv")) rc = -1;
-	if (sf_params()->enable_tree && c->N && sf_tree_check(&sim_ptr->tree, c->stream)) rc = -1;
+	if (sf_params()->enable_tree && c->N && sf_tree_check(&sim_ptr->tree, c->stream, "sim_solv")) rc = -1;
 	return rc ? STARFLOOD_FAILURE : STARFLOOD_SUCCESS;
 }
 
@@ -288,6 +288,8 @@ int sf_sim_step(sim_t* SF_RESTRICT sim_ptr) {
 		SF_HIP(hipMemcpyAsync(c->h_red, red_out, SF_RED_VALUES * sizeof(double), hipMemcpyDeviceToHost, s),
 		       "sim_step");
 		if (sf_finish(c, "sim_step")) return STARFLOOD_FAILURE;
+		/* the step synchronises here anyway: report a tree walk cut short */
+		if (p->enable_tree && N && sf_tree_check(&sim_ptr->tree, s, "sim_step")) rc = -1;
 		sf_stats_t* st = &c->stats;
 		const double* r = c->h_red;
 		st->T = exact ? r[12] : r[0];
@@ -358,6 +360,8 @@ int sf_get_stats(const sim_t* sim_ptr, sf_stats_t* out) {
 		out->L[k] = r[7 + k];
 	}
 	out->K = r[10];
+	/* a tree walk cut short since the last check fails the call */
+	if (p->enable_tree && N && sf_tree_check((tree_t*)&sim_ptr->tree, s, "sf_get_stats")) return STARFLOOD_FAILURE;
 	return STARFLOOD_SUCCESS;
 }
 

@@ -314,23 +314,22 @@ int sf_lbvh_export(const tree_t* tree_ptr, unsigned long long* keys, unsigned in
 	                                                                                           : STARFLOOD_SUCCESS;
 }
 
-/* After an LBVH walk on stream s: fail if any wavefront met a malformed
-   tree (synchronises s). */
-static int lbvh_check(sf_tree_dev* d, hipStream_t s) {
-	unsigned long long counts[4] = {0, 0, 0, 0};
-	if (d->lbvh && sf_lbvh_size(d->lbvh) == 0u) return 0;   /* no particles: nothing was walked */
-	if (!d->lbvh || sf_lbvh_export_impl(d->lbvh, s, NULL, NULL, counts, NULL, NULL)) return -1;
-	if (counts[2]) {
-		sf_error("solve_gravity_part_tree", "%llu wavefronts met a malformed tree", counts[2]);
+/* After LBVH walks on stream s: fail if any wavefront of the walks since
+   the last check met a malformed tree or ran out of stack (synchronises s). */
+static int lbvh_check(sf_tree_dev* d, hipStream_t s, const char* fn) {
+	unsigned long long tripped = 0;
+	if (sf_lbvh_guard(d->lbvh, s, &tripped)) return -1;
+	if (tripped) {
+		sf_error(fn, "%llu wavefronts of the tree walk met a malformed tree: forces are incomplete", tripped);
 		return -1;
 	}
 	return 0;
 }
 
-int sf_tree_check(tree_t* tree, hipStream_t s) {
+int sf_tree_check(tree_t* tree, hipStream_t s, const char* fn) {
 	if (sf_params()->tree_mode != SF_TREE_LBVH) return 0;
-	sf_tree_dev* d = tree_dev_lookup(tree, "sim_solv");
-	return d ? lbvh_check(d, s) : -1;
+	sf_tree_dev* d = tree_dev_lookup(tree, fn);
+	return d ? lbvh_check(d, s, fn) : -1;
 }
 
 /* reference src/gravity.c:245-663: walks the tree last built for this
@@ -357,7 +356,7 @@ static int gravity_part_tree(const sim_t* SF_RESTRICT sim_ptr) {
 	} else {
 		rc = d->lbvh ? sf_lbvh_walk(d->lbvh, N, pos, mas, p, acc, pot, s) : -1;
 		if (!d->lbvh) sf_error("solve_gravity_part_tree", "no tree was built");
-		if (!rc && lbvh_check(d, s)) rc = -1;
+		if (!rc && lbvh_check(d, s, "solve_gravity_part_tree")) rc = -1;
 	}
 	if (sf_download_blocks(c, SF_E_ACC, 3u) || sf_download_blocks(c, SF_E_POT, 1u)) rc = -1;
 	if (sf_finish(c, "solve_gravity_part_tree")) rc = -1;

@@ -165,6 +165,17 @@ class Simulation:
                                           pot.ctypes.data), "sf_pp_direct_f64")
         return acc, pot
 
+    def sph_direct_f64(self, idx: np.ndarray) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """FP64 cubic-model rho (n,), SPH acc (n,3) and du/dt (n,) for the
+        listed particles."""
+        idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        rho = np.zeros(len(idx), np.float64)
+        acc = np.zeros((len(idx), 3), np.float64)
+        du = np.zeros(len(idx), np.float64)
+        L.check(self.lib.sf_sph_direct_f64(C.byref(self.sim), idx.ctypes.data, len(idx), rho.ctypes.data,
+                                           acc.ctypes.data, du.ctypes.data), "sf_sph_direct_f64")
+        return rho, acc, du
+
     def lbvh_export(self, nodes: bool = False) -> dict:
         """Sorted Morton codes, permutation and walk counters of the last
         SF_TREE_LBVH build (and node arrays when nodes=True)."""

@@ -1,0 +1,287 @@
+"""The path on the device at the full size of each configured workload,
+through the C ABI, with its accuracy stated against FP64:
+
+  config 2  PP gravity + cubic SPH gas ball, N = 65536, 50 KDK steps (both
+            pair searches): energy and momentum drift; gravity against the
+            FP64 direct sum and SPH against the FP64 oracle on sampled rows
+  config 3  Barnes-Hut, Plummer N = 1048576, theta 0.5, eps 0.005, 20 steps:
+            force error percentiles against the FP64 direct sum
+  config 4  adjoint KDK, two Plummer spheres N = 16384, 200 steps, a
+            checkpoint every 20: the gradient against autograd of a plain
+            FP64 forward and against FP64 central differences
+  config 5  PP N = 262144 and Barnes-Hut N = 4194304 at theta 0.3 / 0.5 / 0.7
+            (and quadrupole cells at 0.55), eps 0.002, 10 steps: error
+            percentiles and energy drift
+
+The FP64 direct sum on the device (sf_pp_direct_f64) is pinned to the CPU
+oracle by test_gpu_pp.py; the bounds below are stated per test."""
+import numpy as np
+import pytest
+import torch
+
+from starflood_amd import _lib as L
+from starflood_amd.sim import Simulation
+
+pytestmark = pytest.mark.gpu
+
+FAST = dict(logging=0, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL, mem_mode=L.SF_MEM_RESIDENT)
+
+
+def rel_err(acc, ref):
+    return np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+
+
+def pcts(e):
+    return {q: float(np.percentile(e, q)) for q in (50, 90, 99)} | {"max": float(e.max())}
+
+
+def kdk(s, steps):
+    for n in range(steps):
+        s.step_number = n
+        s.step()
+
+
+@pytest.fixture(autouse=True)
+def _restore_params():
+    yield
+    L.set_params()
+
+
+# ---------------------------------------------------------------- config 2
+@pytest.mark.parametrize("search", [L.SF_SPH_SEARCH_GRID, L.SF_SPH_SEARCH_ALL], ids=["grid", "all_pairs"])
+def test_config2_gas_ball_n65536_50_steps(oracle, search):
+    """Uniform gas ball N = 65536 (v = 0, m = 1/N, gamma 5/3, u = 0.05,
+    h = 0.05, eps = 0.01, dt = 5e-4), 50 KDK steps.  E = K + W + U drifts by
+    at most 1e-6 relative (measured ~1e-8) and the momentum stays below 1e-9
+    (FP64 sums of FP32 terms).  On the final state, every 64th particle:
+    the fast PP gravity within 1e-5 of the FP64 direct sum; rho and P within
+    2e-5, the pressure acceleration and du/dt within 1e-4 (of the
+    per-particle magnitude) of the FP64 oracle."""
+    N = 65536
+    print(f"config 2: N={N} search={search}", flush=True)
+    cfg = dict(enable_tree=0, enable_sph=1, sph_model=L.SF_SPH_CUBIC_IDEAL, sph_gamma=5.0 / 3.0, epsilon=1e-2,
+               dt=5e-4, sph_search=search, **FAST)
+    with Simulation(N, **cfg) as s:
+        s.ic_uniform_ball(seed=41, radius=1.0, total_mass=1.0, v_hubble=0.0)
+        s.rad[:] = 0.05
+        s.vel[:] = 0.0
+        s.to_device()
+        s.set_u(np.full(N, 0.05, np.float32))
+        s.solve()
+        e0 = s.stats()
+        kdk(s, 50)
+        e1 = s.stats()
+        E0, E1 = e0["K"] + e0["W"] + e0["U"], e1["K"] + e1["W"] + e1["U"]
+        print(f"  energy drift {abs(E1 - E0) / abs(E0):.3e}, |P| {np.abs(e1['P']).max():.3e}", flush=True)
+        assert abs(E1 - E0) <= 1e-6 * abs(E0)
+        assert np.abs(e1["P"]).max() <= 1e-9
+        assert abs(e1["U"] - e0["U"]) > 1e-4 * e0["U"]     # pressure did work (the edge expands)
+        s.to_host()
+        pos, vel, mas, rad = s.pos.copy(), s.vel.copy(), s.mas.copy(), s.rad.copy()
+        u = s.get_u()
+    rows = np.arange(0, N, 64, dtype=np.uint32)
+    # gravity of the final state against the FP64 direct sum
+    with Simulation(N, enable_tree=0, epsilon=1e-2, **FAST) as g:
+        g.pos[:] = pos
+        g.mas[:] = mas
+        g.to_device()
+        g.solve_gravity_pp()
+        g.to_host()
+        ref, _ = g.direct_f64(rows)
+        e = rel_err(g.acc[rows].astype(np.float64), ref)
+        assert e.max() <= 1e-5
+    # SPH of the final state against the FP64 oracle, sampled rows
+    rho_o, prs_o, acc_o, du_o = oracle.sph_cubic_rows_f64(pos, vel, mas, rad, u, 5.0 / 3.0, rows)
+    with Simulation(N, **(cfg | dict(mem_mode=L.SF_MEM_MIRROR))) as h:
+        h.pos[:] = pos
+        h.vel[:] = vel
+        h.mas[:] = mas
+        h.rad[:] = rad
+        h.acc[:] = 0
+        h.set_u(u)
+        h.solve_sph()
+        assert np.max(np.abs(h.rho - rho_o) / rho_o) <= 2e-5
+        assert np.max(np.abs(h.prs - prs_o) / prs_o) <= 2e-5
+        a = h.acc[rows].astype(np.float64)
+        na = np.linalg.norm(acc_o, axis=1)
+        ea = np.linalg.norm(a - acc_o, axis=1) / np.maximum(na, 1e-3 * na.max())
+        du = h.get_dudt()[rows].astype(np.float64)
+        ed = np.abs(du - du_o) / np.maximum(np.abs(du_o), 1e-3 * np.abs(du_o).max())
+        print(f"  SPH acc err max {ea.max():.2e}, du/dt err max {ed.max():.2e}", flush=True)
+        assert ea.max() <= 1e-4
+        assert ed.max() <= 1e-4
+
+
+# ---------------------------------------------------------------- configs 3 and 5 (Barnes-Hut)
+def bh_run(N, steps, eps, seed, stride, **tree):
+    """steps KDK steps of the LBVH tree from a Plummer sphere (a = 1, M = 1);
+    error percentiles of the final accelerations against the FP64 direct
+    sum on every stride-th particle, and the relative drift of E = K + W."""
+    with Simulation(N, enable_tree=1, tree_mode=L.SF_TREE_LBVH, epsilon=eps, dt=1e-3, **tree, **FAST) as s:
+        s.ic_plummer(seed=seed, first=0, count=N)
+        s.solve()
+        e0 = s.stats()
+        kdk(s, steps)
+        e1 = s.stats()
+        s.to_host()
+        idx = np.arange(0, N, stride, dtype=np.uint32)
+        ref, _ = s.direct_f64(idx)
+        err = pcts(rel_err(s.acc[idx].astype(np.float64), ref))
+    E0, E1 = e0["K"] + e0["W"], e1["K"] + e1["W"]
+    drift = abs(E1 - E0) / abs(E0)
+    print(f"  N={N} {tree}: err {err}, energy drift {drift:.3e}", flush=True)
+    return err, drift
+
+
+def test_config3_bh_plummer_1m_20_steps():
+    """Plummer N = 1048576, theta = 0.5 (monopole, default leaf size),
+    eps = 0.005, dt = 1e-3, 20 KDK steps; error on every 256th particle
+    (4096): p50 <= 1e-3, p99 <= 5e-3 (monopole cells at theta 0.5; measured
+    p50 6.9e-4, p99 3.5e-3); relative drift of E = K + W <= 1e-5 (measured
+    3.3e-6: the tree's force is not exactly conservative)."""
+    print("config 3: BH N=1048576 theta=0.5 20 steps", flush=True)
+    err, drift = bh_run(1048576, 20, 5e-3, 3, 256, theta=0.5)
+    assert err[50] <= 1e-3 and err[99] <= 5e-3
+    assert drift <= 1e-5
+
+
+BH4M_BOUNDS = {  # theta: (p50, p99); monopole cells, default leaf size
+    0.3: (3e-4, 1e-3),
+    0.5: (1e-3, 5e-3),
+    0.7: (3e-3, 1.2e-2),
+}
+
+
+def test_config5_bh_4m_theta_sweep():
+    """Plummer N = 4194304, eps = 0.002, 10 KDK steps at theta 0.3, 0.5 and
+    0.7: error on every 1024th particle within the stated percentiles,
+    growing with theta; energy drift <= 1e-5.  Quadrupole cells at theta
+    0.55 keep p99 <= 1e-3."""
+    N = 4194304
+    p50 = []
+    for th, (b50, b99) in BH4M_BOUNDS.items():
+        print(f"config 5: BH N={N} theta={th} 10 steps", flush=True)
+        err, drift = bh_run(N, 10, 2e-3, 5, 1024, theta=th)
+        assert err[50] <= b50 and err[99] <= b99, (th, err)
+        assert drift <= 1e-5, (th, drift)
+        p50.append(err[50])
+    assert p50[0] < p50[1] < p50[2]
+    print(f"config 5: BH N={N} quadrupole theta=0.55 10 steps", flush=True)
+    err, drift = bh_run(N, 10, 2e-3, 5, 1024, theta=0.55, leaf_size=8, quadrupole=1)
+    assert err[99] <= 1e-3 and drift <= 1e-5
+
+
+def test_config5_pp_262144_10_steps():
+    """PP gravity N = 262144 (Plummer, eps = 0.002), 10 KDK steps in the fast
+    policy: every acceleration of every 64th particle within 1e-5 of the
+    FP64 direct sum, energy drift <= 1e-7 (measured ~5e-9)."""
+    N = 262144
+    print(f"config 5: PP N={N} 10 steps", flush=True)
+    with Simulation(N, enable_tree=0, epsilon=2e-3, dt=1e-3, **FAST) as s:
+        s.ic_plummer(seed=5, first=0, count=N)
+        s.solve()
+        e0 = s.stats()
+        kdk(s, 10)
+        e1 = s.stats()
+        s.to_host()
+        idx = np.arange(0, N, 64, dtype=np.uint32)
+        ref, _ = s.direct_f64(idx)
+        e = rel_err(s.acc[idx].astype(np.float64), ref)
+    E0, E1 = e0["K"] + e0["W"], e1["K"] + e1["W"]
+    print(f"  err {pcts(e)}, energy drift {abs(E1 - E0) / abs(E0):.3e}", flush=True)
+    assert e.max() <= 1e-5
+    assert abs(E1 - E0) <= 1e-7 * abs(E0)
+
+
+# ---------------------------------------------------------------- config 4 (adjoint)
+EPS4, DT4, STEPS4, CKPT4 = 1e-2, 1e-3, 200, 20
+
+
+def _acc64(x, m):
+    """FP64 direct-sum acceleration on the device, dense form: r_ij^2 from
+    |x_i|^2 + |x_j|^2 - 2 x_i.x_j (in FP64 its rounding, ~1e-15 here, is
+    1e-11 of eps^2) and sum_j w_ij (x_j - x_i) = W x - x rowsum(W)."""
+    x2 = (x * x).sum(1)
+    s2 = torch.addmm((x2[:, None] + x2[None, :]) + EPS4 * EPS4, x, x.t().contiguous(), alpha=-2.0)
+    w = m[None, :] * s2.pow(-1.5)
+    return w @ x - x * w.sum(1, keepdim=True)
+
+
+def _kdk64(x, v, m, steps, checkpoint=False):
+    """A plain FP64 KDK (torch, device); with checkpoint the force of each
+    step is recomputed in the backward, so autograd through 200 steps fits."""
+    from torch.utils.checkpoint import checkpoint as ck
+    f = (lambda y: ck(_acc64, y, m, use_reentrant=False)) if checkpoint else (lambda y: _acc64(y, m))
+    h = 0.5 * DT4
+    a = f(x)
+    for _ in range(steps):
+        v = v + h * a
+        x = x + DT4 * v
+        a = f(x)
+        v = v + h * a
+    return x, v
+
+
+def test_config4_fp64_torch_forward_matches_oracle(oracle):
+    """The FP64 torch forward used below against the oracle's FP64 KDK."""
+    from test_gpu_adjoint import two_spheres
+    x0, v0, m = two_spheres(256, 21)
+    d = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")
+    xt, vt = _kdk64(d(x0), d(v0), d(m), 20)
+    xo, vo = oracle.kdk_f64(x0, v0, m, EPS4, 1.0, DT4, 1, 20)
+    assert np.abs(xt.cpu().numpy() - xo).max() <= 1e-11
+    assert np.abs(vt.cpu().numpy() - vo).max() <= 1e-10
+
+
+def test_config4_adjoint_two_spheres_n16384_200_steps():
+    """Two Plummer spheres (8192 each, a = 0.5, centres +-1.5 on x, approach
+    speed 0.3; eps 0.01, dt 1e-3), 200 steps, a checkpoint every 20; loss
+    L = mean((x_N - x_t)^2), x_t from the ICs perturbed by N(0, 1e-3).  The
+    device adjoint is driven with the FP64 forward's cotangent (the FP32
+    forward's rounding stays out of the comparison).  Its gradient agrees
+    with autograd of the plain FP64 forward within 1e-4 (norm), and with
+    FP64 central differences along two random directions within 1e-4."""
+    from starflood_amd import adjoint as A
+    N = 16384
+    print(f"config 4: adjoint N={N} steps={STEPS4} ckpt={CKPT4}", flush=True)
+    with Simulation(N, enable_tree=0, logging=0, epsilon=EPS4, dt=DT4) as s:
+        s.ic_plummer(seed=31, first=0, count=N // 2, a=0.5, M=0.5, centre=(-1.5, 0, 0), bulk_v=(0.15, 0, 0))
+        s.ic_plummer(seed=32, first=N // 2, count=N // 2, a=0.5, M=0.5, centre=(1.5, 0, 0), bulk_v=(-0.15, 0, 0))
+        x0, v0, m = s.pos.copy(), s.vel.copy(), s.mas.copy()
+    L.set_params(epsilon=EPS4, dt=DT4, G=1.0, acc_mode=L.SF_ACC_PHYSICAL)
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(33)
+    d64 = lambda a: torch.tensor(np.asarray(a, np.float64), device=dev)
+    x64, v64, m64 = d64(x0), d64(v0), d64(m)
+    with torch.no_grad():
+        xt, _ = _kdk64(d64(x0 + rng.normal(scale=1e-3, size=x0.shape)), v64, m64, STEPS4)
+    loss = lambda xN: ((xN - xt) ** 2).mean()
+    # reference gradient: autograd through the FP64 forward
+    rx, rv = x64.clone().requires_grad_(True), v64.clone().requires_grad_(True)
+    xN64, _ = _kdk64(rx, rv, m64, STEPS4, checkpoint=True)
+    L64 = loss(xN64)
+    gxN = torch.autograd.grad(L64, xN64, retain_graph=True)[0]
+    L64.backward()
+    gx_ref, gv_ref = rx.grad.cpu().numpy(), rv.grad.cpu().numpy()
+    # the device adjoint (FP32 forward with FP64 drift, checkpoints every 20)
+    tx = torch.from_numpy(x0).to(dev).requires_grad_(True)
+    tv = torch.from_numpy(v0).to(dev).requires_grad_(True)
+    xN, _ = A.kdk(tx, tv, torch.from_numpy(m).to(dev), STEPS4, CKPT4)
+    assert (xN.detach().double() - xN64.detach()).abs().max().item() <= 1e-5
+    xN.backward(gxN.float())
+    gx, gv = tx.grad.cpu().numpy().astype(np.float64), tv.grad.cpu().numpy().astype(np.float64)
+    ex = np.linalg.norm(gx - gx_ref) / np.linalg.norm(gx_ref)
+    ev = np.linalg.norm(gv - gv_ref) / np.linalg.norm(gv_ref)
+    print(f"  gradient vs FP64 autograd: x {ex:.2e}, v {ev:.2e}", flush=True)
+    assert ex <= 1e-4 and ev <= 1e-4
+    # FP64 central differences along random directions in (x0, v0)
+    hstep = 1e-5
+    for k in range(2):
+        dx, dv = rng.normal(size=x0.shape), rng.normal(size=v0.shape)
+        with torch.no_grad():
+            lp = loss(_kdk64(x64 + hstep * d64(dx), v64 + hstep * d64(dv), m64, STEPS4)[0]).item()
+            lm = loss(_kdk64(x64 - hstep * d64(dx), v64 - hstep * d64(dv), m64, STEPS4)[0]).item()
+        fd = (lp - lm) / (2.0 * hstep)
+        dd = float((gx * dx).sum() + (gv * dv).sum())
+        print(f"  direction {k}: adjoint {dd:.9e}, central difference {fd:.9e}", flush=True)
+        assert abs(dd - fd) <= 1e-4 * abs(fd)

@@ -197,6 +197,29 @@ def test_cubic_gas_sphere_conserves_energy_and_momentum():
         assert np.abs(e1["P"]).max() <= 1e-6
 
 
+def test_fp64_sph_reference_matches_the_oracle(oracle):
+    """sf_sph_direct_f64, the device's FP64 precision reference for the
+    cubic model (used by the benchmark), against the CPU oracle: rho, the
+    pressure acceleration and du/dt within 1e-12 relative (FP64 sums in a
+    different order), mixed smoothing lengths."""
+    N = 4096
+    pos, vel, mas, rad, u = cubic_state(N, 5, 0.15)
+    rad[::7] *= 1.5
+    rows = np.arange(1, N, 3, dtype=np.uint32)
+    rho_o, _, acc_o, du_o = oracle.sph_cubic_f64(pos, vel, mas, rad, u, 5.0 / 3.0)
+    with cubic_sim(N, mem_mode=L.SF_MEM_MIRROR) as s:
+        s.pos[:] = pos
+        s.vel[:] = vel
+        s.mas[:] = mas
+        s.rad[:] = rad
+        s.set_u(u)
+        rho, acc, du = s.sph_direct_f64(rows)
+    assert np.max(np.abs(rho - rho_o[rows]) / rho_o[rows]) <= 1e-12
+    na = np.linalg.norm(acc_o[rows], axis=1)
+    assert np.max(np.linalg.norm(acc - acc_o[rows], axis=1) / np.maximum(na, 1e-6 * na.max())) <= 1e-12
+    assert np.max(np.abs(du - du_o[rows]) / np.maximum(np.abs(du_o[rows]), 1e-6 * np.abs(du_o).max())) <= 1e-12
+
+
 @pytest.mark.parametrize("search", [L.SF_SPH_SEARCH_GRID, L.SF_SPH_SEARCH_ALL])
 def test_cubic_model_lattice_closed_forms(search):
     """The closed forms that pin the oracle's pressure force and du/dt, on

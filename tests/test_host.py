@@ -53,6 +53,11 @@ def test_sf_run_rejects_bad_configurations_before_any_device_work(tmp_path):
     assert lib.sf_run(C.byref(cfg)) == L.STARFLOOD_FAILURE
     cfg = L.RunConfig(num_bodies=16, num_timesteps=1, enable_sim=1, vis_file_type=7)
     assert lib.sf_run(C.byref(cfg)) == L.STARFLOOD_FAILURE
+    # the cubic SPH model's u is not carried by the run loop or its snapshots
+    L.set_params(out_dir=str(tmp_path), enable_sph=1, sph_model=L.SF_SPH_CUBIC_IDEAL)
+    cfg = L.RunConfig(num_bodies=16, num_timesteps=1, enable_sim=1)
+    assert lib.sf_run(C.byref(cfg)) == L.STARFLOOD_FAILURE
+    assert "internal energy" in L.last_error()
     assert list(tmp_path.iterdir()) == []
     L.set_params()
 

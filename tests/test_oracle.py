@@ -349,3 +349,21 @@ def test_morton_codes_match_magic_number_interleave(oracle):
     # of the first bisection
     oct_ = ((q[:, 0] >> np.uint64(20)) << np.uint64(2)) | ((q[:, 1] >> np.uint64(20)) << np.uint64(1)) | (q[:, 2] >> np.uint64(20))
     assert np.array_equal(key >> np.uint64(60), oct_)
+
+
+def test_cubic_sph_rows_variant_equals_full_evaluation(oracle):
+    """The sampled-rows form (used at N = 65536, where a full FP64 force pass
+    is too slow) runs the same per-row loop: bit-identical rows."""
+    rng = np.random.default_rng(0)
+    N = 3000
+    pos = rng.normal(size=(N, 3)) * 0.3
+    vel = rng.normal(size=(N, 3))
+    mas = np.full(N, 1.0 / N)
+    h = np.full(N, 0.2)
+    h[::5] = 0.3
+    u = rng.uniform(0.02, 0.08, N)
+    rho, prs, acc, du = oracle.sph_cubic_f64(pos, vel, mas, h, u, 5.0 / 3.0)
+    rows = np.arange(5, N, 37)
+    r_rho, r_prs, r_acc, r_du = oracle.sph_cubic_rows_f64(pos, vel, mas, h, u, 5.0 / 3.0, rows)
+    assert np.array_equal(rho, r_rho) and np.array_equal(prs, r_prs)
+    assert np.array_equal(acc[rows], r_acc) and np.array_equal(du[rows], r_du)
