@@ -240,6 +240,18 @@ void sf_tape_free(sf_tape* tape);
 int sf_force_dev(unsigned int N, const float* x, const float* m, float* acc, float* pot, void* stream);
 /* Force vector-Jacobian product: g = (d acc / d x)^T mu, device arrays. */
 int sf_force_vjp_dev(unsigned int N, const float* x, const float* m, const float* mu, float* g, void* stream);
+/* sf_force_dev without allocation.  No counterpart in the reference: it is
+   for callers that cannot allocate device memory (code under stream
+   capture, allocation-free serving loops).  Same kernels and summation
+   order as sf_force_dev, so the same acc and pot bit for bit; its scratch
+   (packed bodies and j-slice partials, 32.3 MiB) is a static device array
+   that each GPU the library runs on holds until the process ends.  N above
+   SF_NOALLOC_MAX_N fails with an error.  The work is queued on `stream`
+   and the call returns without waiting for it.  Every call on one GPU
+   shares that GPU's scratch: calls that overlap on one GPU (other streams,
+   other host threads) race. */
+#define SF_NOALLOC_MAX_N 16384u
+int sf_force_dev_noalloc(unsigned int N, const float* x, const float* m, float* acc, float* pot, void* stream);
 
 /* --- the reference program's run loop (src/main.c:348-611) ---
    The reference sets these with config.h macros; the physics parameters

@@ -29,6 +29,7 @@ SF_SPH_SEARCH_ALL, SF_SPH_SEARCH_GRID = 0, 1
 SF_MEM_MIRROR, SF_MEM_RESIDENT = 0, 1
 SF_SYNC_TO_DEVICE, SF_SYNC_TO_HOST = 0, 1
 SF_EXT_U, SF_EXT_DUDT, SF_EXT_X64 = 0, 1, 2
+SF_NOALLOC_MAX_N = 16384   # include/starflood_ext.h
 
 
 class NodeT(C.Structure):
@@ -129,6 +130,7 @@ _SIG = [
     ("sf_tape_free", None, [C.c_void_p]),
     ("sf_force_dev", C.c_int, [C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("sf_force_vjp_dev", C.c_int, [C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("sf_force_dev_noalloc", C.c_int, [C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("sf_run", C.c_int, [C.POINTER(RunConfig)]),
 ]
 
