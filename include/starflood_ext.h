@@ -244,7 +244,7 @@ int sf_force_vjp_dev(unsigned int N, const float* x, const float* m, const float
    for callers that cannot allocate device memory (code under stream
    capture, allocation-free serving loops).  Same kernels and summation
    order as sf_force_dev, so the same acc and pot bit for bit; its scratch
-   (packed bodies and j-slice partials, 32.3 MiB) is a static device array
+   (packed bodies and j-slice partials, 8.3 MiB) is a static device array
    that each GPU the library runs on holds until the process ends.  N above
    SF_NOALLOC_MAX_N fails with an error.  The work is queued on `stream`
    and the call returns without waiting for it.  Every call on one GPU

@@ -6,9 +6,9 @@
 //
 // scratch holds the packed bodies (n_pad float4, then the pack kernel's
 // SF_BODY_STATE words) and, from the next 256-byte boundary, the slice
-// partials (slices x 2 planes x n_pad float4).  It is sized for N up to
-// SF_NOALLOC_MAX_N, where n_pad = N and the fast kernel uses at most 64 j
-// slices: 256 KiB + 256 B of bodies and 32 MiB of partials.  Each device
+// partials (one double4 per particle and block slice).  It is sized for N up
+// to SF_NOALLOC_MAX_N, where n_pad = N and the fast kernel writes 16 block
+// slices: 256 KiB + 256 B of bodies and 8 MiB of partials.  Each device
 // holds its own copy; its address comes from hipGetSymbolAddress, once per
 // device, on the first call with that device current.
 #include <hip/hip_runtime.h>
@@ -18,20 +18,20 @@
 #include "../sf_internal.h"
 
 extern "C" {
-unsigned int sf_pp_slice_count(unsigned int n_pad);
+size_t sf_pp_part_bytes(unsigned int n_pad);
 int sf_pp_force_arrays(const sf_params_t* p, unsigned int N, unsigned int n_pad, const float* pos, const float* mas,
-                       float* body, float* part, unsigned int part_slices, float* acc, float* pot, hipStream_t s);
+                       float* body, float* part, size_t part_bytes, float* acc, float* pot, hipStream_t s);
 int sfd_fill0(float* y, unsigned int n, hipStream_t s);
 }
 
 namespace {
 
 constexpr unsigned kMaxN = SF_NOALLOC_MAX_N;
-constexpr unsigned kMaxSlices = 64;
+constexpr unsigned kMaxBlockSlices = 16;   // of k_pp_fast at n_pad = kMaxN
 constexpr size_t kAlign = 256;
 constexpr size_t align_up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
 constexpr size_t kBodyBytes = align_up(SF_BODY_BYTES(kMaxN));
-constexpr size_t kPartBytes = (size_t)kMaxSlices * 2u * kMaxN * 16u;
+constexpr size_t kPartBytes = (size_t)kMaxBlockSlices * kMaxN * 32u;
 
 static __device__ __align__(256) unsigned char scratch[kBodyBytes + kPartBytes];
 
@@ -73,12 +73,11 @@ extern "C" int sf_force_dev_noalloc(unsigned int N, const float* x, const float*
 	}
 	// the layout sf_force_dev allocates: n_pad a multiple of 1024
 	const unsigned n_pad = (N + 1023u) / 1024u * 1024u;
-	const unsigned slices = sf_pp_slice_count(n_pad);
 	const size_t body_bytes = align_up(SF_BODY_BYTES(n_pad));
-	const size_t part_bytes = (size_t)slices * 2u * n_pad * 16u;
+	const size_t part_bytes = sf_pp_part_bytes(n_pad);
 	if (body_bytes + part_bytes > sizeof(scratch)) {
-		sf_error(fn, "N = %u with %u j slices needs %zu scratch bytes, more than the %zu available", N, slices,
-		         body_bytes + part_bytes, sizeof(scratch));
+		sf_error(fn, "N = %u needs %zu scratch bytes, more than the %zu available", N, body_bytes + part_bytes,
+		         sizeof(scratch));
 		return STARFLOOD_FAILURE;
 	}
 	unsigned char* base = scratch_ptr(fn);
@@ -90,6 +89,6 @@ extern "C" int sf_force_dev_noalloc(unsigned int N, const float* x, const float*
 	// with N and may hold an earlier call's bodies: clear them (a kernel, not
 	// a memset node, so the call can be captured).
 	if (sfd_fill0(body + 4u * (size_t)n_pad, SF_BODY_STATE, s)) return STARFLOOD_FAILURE;
-	return sf_pp_force_arrays(sf_params(), N, n_pad, x, m, body, part, slices, acc, pot, s) ? STARFLOOD_FAILURE
-	                                                                                       : STARFLOOD_SUCCESS;
+	return sf_pp_force_arrays(sf_params(), N, n_pad, x, m, body, part, part_bytes, acc, pot, s) ? STARFLOOD_FAILURE
+	                                                                                           : STARFLOOD_SUCCESS;
 }

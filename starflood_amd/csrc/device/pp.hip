@@ -371,6 +371,27 @@ extern "C" int sfd_pp_exact(const float* pos, const float* mas, unsigned N, floa
 	return sf_hip_check(hipGetLastError(), "sfd_pp_exact", "k_pp_exact launch");
 }
 
+// The slice geometry of k_pp_fast for n_pad bodies and `slices` j ranges
+// (whole SF_TJ tiles, every one non-empty): sub-slice length, and block
+// slices (kWavesPP sub-slices each), one double4 plane each.
+struct PPGeometry {
+	unsigned sub_len, bslices;
+};
+static PPGeometry pp_geometry(unsigned n_pad, unsigned slices) {
+	const unsigned tiles = n_pad / SF_TJ;
+	if (slices > tiles) slices = tiles;
+	if (slices < 1u) slices = 1u;
+	PPGeometry g;
+	g.sub_len = (tiles + slices - 1) / slices * SF_TJ;
+	const unsigned subs = (n_pad + g.sub_len - 1) / g.sub_len;
+	g.bslices = (subs + kWavesPP - 1) / kWavesPP;
+	return g;
+}
+
+extern "C" size_t sfd_pp_part_bytes(unsigned n_pad, unsigned slices) {
+	return n_pad ? (size_t)pp_geometry(n_pad, slices).bslices * n_pad * sizeof(double4) : 0u;
+}
+
 extern "C" int sfd_pp_fast(const float* body, unsigned N, unsigned n_pad, float eps2, double G, int acc_phys,
                            float* part, unsigned slices, float* acc, float* pot, hipStream_t s) {
 	if (N == 0) return 0;
@@ -378,29 +399,22 @@ extern "C" int sfd_pp_fast(const float* body, unsigned N, unsigned n_pad, float 
 		sf_error("sfd_pp_fast", "n_pad %u is not a multiple of %u", n_pad, SF_TJ);
 		return -1;
 	}
-	// `slices` j ranges (whole SF_TJ tiles, every one non-empty), one per
-	// wavefront of a block: slices / kWavesPP block slices, each writing one
-	// double4 plane, so part needs ceil(slices / kWavesPP) * n_pad * 32 bytes
-	// (no more than the slices x 2 float4 planes callers reserve)
-	const unsigned tiles = n_pad / SF_TJ;
-	if (slices > tiles) slices = tiles;
-	if (slices < 1u) slices = 1u;
-	const unsigned sub_len = (tiles + slices - 1) / slices * SF_TJ;
-	const unsigned subs = (n_pad + sub_len - 1) / sub_len;
-	const unsigned bslices = (subs + kWavesPP - 1) / kWavesPP;
-	const unsigned grid = (n_pad / 64u) * bslices;
+	// part holds sfd_pp_part_bytes(n_pad, slices): one double4 plane per block slice
+	const PPGeometry g = pp_geometry(n_pad, slices);
+	const unsigned grid = (n_pad / 64u) * g.bslices;
 	double4* p4 = reinterpret_cast<double4*>(part);
 	const float4* b4 = reinterpret_cast<const float4*>(body);
 	const int tok = sf_prof_begin("pp_fast", s);
 	if (pot)
-		hipLaunchKernelGGL(k_pp_fast<true>, dim3(grid), dim3(kBlock), 0, s, b4, n_pad, bslices, sub_len, eps2, p4);
+		hipLaunchKernelGGL(k_pp_fast<true>, dim3(grid), dim3(kBlock), 0, s, b4, n_pad, g.bslices, g.sub_len, eps2, p4);
 	else
-		hipLaunchKernelGGL(k_pp_fast<false>, dim3(grid), dim3(kBlock), 0, s, b4, n_pad, bslices, sub_len, eps2, p4);
+		hipLaunchKernelGGL(k_pp_fast<false>, dim3(grid), dim3(kBlock), 0, s, b4, n_pad, g.bslices, g.sub_len, eps2,
+		                   p4);
 	sf_prof_end(tok, s);
 	if (sf_hip_check(hipGetLastError(), "sfd_pp_fast", "k_pp_fast launch")) return -1;
 	const int tok2 = sf_prof_begin("pp_combine", s);
-	hipLaunchKernelGGL(k_pp_combine, dim3((N + kCombI - 1) / kCombI), dim3(kBlock), 0, s, p4, n_pad, bslices, b4, N,
-	                   G, acc_phys, acc, pot);
+	hipLaunchKernelGGL(k_pp_combine, dim3((N + kCombI - 1) / kCombI), dim3(kBlock), 0, s, p4, n_pad, g.bslices, b4,
+	                   N, G, acc_phys, acc, pot);
 	sf_prof_end(tok2, s);
 	return sf_hip_check(hipGetLastError(), "sfd_pp_fast", "k_pp_combine launch");
 }

@@ -71,7 +71,8 @@ static float eps2_of(const sf_params_t* p) { return (float)(p->epsilon * p->epsi
 
 static int force(const sf_params_t* p, sf_pp_scratch* w, const float* x, const float* m, float* acc, float* pot,
                  hipStream_t s) {
-	return sf_pp_force_arrays(p, w->N, w->n_pad, x, m, w->body, w->part, w->slices, acc, pot, s);
+	return sf_pp_force_arrays(p, w->N, w->n_pad, x, m, w->body, w->part,
+	                          (size_t)w->slices * 2u * w->n_pad * 4u * sizeof(float), acc, pot, s);
 }
 
 /* g (=, or += when accumulate) J(x)^T (coef mu) */

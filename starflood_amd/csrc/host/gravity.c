@@ -10,14 +10,8 @@
 
 /* j slices of the fast kernel: enough blocks to give every CU several waves */
 static unsigned int pp_slices(unsigned int n_pad) {
-	static int env = -2;
-	if (env == -2) {
-		const char* v = getenv("SF_PP_SLICES");
-		env = (v && *v) ? atoi(v) : -1;
-	}
-	if (env > 0) return (unsigned int)env;
-	/* 256 lanes x 1 i per lane per block; about 32k blocks (measured best at
-	   N = 262144: 32 slices) while each slice keeps at least 256 j */
+	/* one j range per wavefront of 64 i; about 128k wavefronts (measured best
+	   at N = 262144: 32 ranges) while each range keeps at least 256 j */
 	const unsigned int iblocks = n_pad / 256u;
 	unsigned int s = iblocks ? (32768u + iblocks - 1u) / iblocks : 1u;
 	const unsigned int smax = n_pad / 256u;
@@ -51,9 +45,14 @@ static int pp_on(const sf_params_t* p, unsigned int N, unsigned int n_pad, const
 
 unsigned int sf_pp_slice_count(unsigned int n_pad) { return pp_slices(n_pad); }
 
+size_t sf_pp_part_bytes(unsigned int n_pad) { return sfd_pp_part_bytes(n_pad, pp_slices(n_pad)); }
+
 int sf_pp_force_arrays(const sf_params_t* p, unsigned int N, unsigned int n_pad, const float* pos, const float* mas,
-                       float* body, float* part, unsigned int part_slices, float* acc, float* pot, hipStream_t s) {
-	if (part_slices < pp_slices(n_pad)) return -1;
+                       float* body, float* part, size_t part_bytes, float* acc, float* pot, hipStream_t s) {
+	if (part_bytes < sf_pp_part_bytes(n_pad)) {
+		sf_error("sf_pp_force_arrays", "%zu bytes of slice partials, %zu needed", part_bytes, sf_pp_part_bytes(n_pad));
+		return -1;
+	}
 	if (sfd_pack_bodies(pos, mas, N, n_pad, body, s)) return -1;
 	return sfd_pp_fast(body, N, n_pad, eps2_f32(p->epsilon), p->G, p->acc_mode == SF_ACC_PHYSICAL, part,
 	                   pp_slices(n_pad), acc, pot, s);

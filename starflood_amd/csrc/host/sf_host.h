@@ -11,12 +11,14 @@
 
 /* gravity.c: PP gravity from d_mem pos/mas into d_mem acc/pot */
 int sf_solve_gravity_pp(sf_ctx* c);
-/* gravity.c: j slices of the fast PP kernel for n_pad particles, and the
-   fast PP force on caller arrays (body: n_pad float4, part: part_slices x 2
-   x n_pad float4, pot may be NULL) */
+/* gravity.c: j ranges of the fast PP kernel (and of the SPH all-pairs and
+   VJP passes) for n_pad particles, the bytes of slice partials the fast PP
+   kernel writes, and the fast PP force on caller arrays (body: n_pad float4,
+   part: part_bytes >= sf_pp_part_bytes(n_pad), pot may be NULL) */
 unsigned int sf_pp_slice_count(unsigned int n_pad);
+size_t sf_pp_part_bytes(unsigned int n_pad);
 int sf_pp_force_arrays(const sf_params_t* p, unsigned int N, unsigned int n_pad, const float* pos, const float* mas,
-                       float* body, float* part, unsigned int part_slices, float* acc, float* pot, hipStream_t s);
+                       float* body, float* part, size_t part_bytes, float* acc, float* pot, hipStream_t s);
 /* tree.c: bounds as src/simulation.c:245-301, build, then walk into acc/pot */
 int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number);
 /* tree.c: after an LBVH solve on stream s, fail if the walk met a malformed

@@ -106,10 +106,12 @@ int sfd_pack_bodies(const float* pos, const float* mas, unsigned int N, unsigned
 int sfd_pp_exact(const float* pos, const float* mas, unsigned int N, float eps2, float negG,
                  int acc_phys, int kahan_acc, int kahan_pot, float* acc, float* pot, hipStream_t s);
 
-/* Fast PP gravity: j-sliced partial sums into part, then a fixed-order FP64
-   combine into acc (AoS) and pot.  pot may be NULL (skip). */
+/* Fast PP gravity: j-sliced partial sums into part (sfd_pp_part_bytes of
+   n_pad and slices), then a fixed-order FP64 combine into acc (AoS) and pot.
+   pot may be NULL (skip). */
 int sfd_pp_fast(const float* body, unsigned int N, unsigned int n_pad, float eps2, double G,
                 int acc_phys, float* part, unsigned int slices, float* acc, float* pot, hipStream_t s);
+size_t sfd_pp_part_bytes(unsigned int n_pad, unsigned int slices);
 
 /* FP64 direct sum for the listed particles (device idx list). */
 int sfd_pp_f64(const float* pos, const float* mas, unsigned int N, double eps2, double G, int acc_phys,
