@@ -44,11 +44,12 @@ DT = 1.0e-3
 SEED = 11
 PEAK_FP32_TFLOPS = 157.3
 FLOPS_PER_INTERACTION = 20
-# HBM bytes of one k_pp_fast dispatch at N = 262144 (32 j slices): rocprofv3
-# FETCH_SIZE 18324 KiB x 2.0 + WRITE_SIZE 262144 KiB x 1.0, the factors from a
-# 1 GiB copy read under the same counters (profiles/pp_fast_n262144.md).  The
-# writes are the slice partials; the kernel is VALU-bound.
-PP_FAST_TRAFFIC_BYTES = (18324 * 2 + 262144) * 1024
+# HBM bytes of one k_pp_fast dispatch at N = 262144 (8 block slices of FP64
+# partials): rocprofv3 FETCH_SIZE 18330 KiB x 2.0 + WRITE_SIZE 65536 KiB x 1.0,
+# the factors from a 1 GiB copy read under the same counters
+# (profiles/pp_fast_n262144.md).  The writes are the slice partials; the
+# kernel is VALU-bound.
+PP_FAST_TRAFFIC_BYTES = (18330 * 2 + 65536) * 1024
 
 
 def hip_sync():
@@ -340,18 +341,23 @@ def rel_err_pcts(acc, ref):
 
 
 BH_RUNS = {
-    # monopole cells: the reference tree's expansion
+    # monopole cells: the reference tree's expansion, at the headline theta
     "monopole_theta0.5": dict(theta=0.5, leaf_size=3, quadrupole=0),
-    # quadrupole cells at the theta whose 99th-percentile force error is below 1e-3
-    "quadrupole_theta0.55": dict(theta=0.55, leaf_size=8, quadrupole=1),
+    # the fastest configuration of a sweep at this N (quadrupole theta 0.55 to
+    # 0.75 at leaf 4, 6, 8; monopole theta 0.3 to 0.5) whose 99th-percentile
+    # force error stays within 1e-3: quadrupole cells, theta 0.6, leaf 4
+    "quadrupole_theta0.6": dict(theta=0.6, leaf_size=4, quadrupole=1),
+    # and with more margin: theta 0.55
+    "quadrupole_theta0.55": dict(theta=0.55, leaf_size=4, quadrupole=1),
 }
 
 
 def bench_bh(steps: int = 3, pp_compare: bool = True) -> dict:
     """Barnes-Hut (SF_TREE_LBVH) KDK steps at N = 4194304 (Plummer a = 1,
-    seeded), monopole at theta = 0.5 and quadrupole at theta = 0.55, against
-    one fast PP force evaluation of the same state; force errors of all
-    three against the FP64 direct sum on every 1024th particle."""
+    seeded): monopole cells at theta = 0.5, and quadrupole cells at theta =
+    0.6 and 0.55 (leaf 4), against one fast PP force evaluation of the same
+    state; force errors of all against the FP64 direct sum on every 1024th
+    particle of each run's final state."""
     N = 4194304
     common = dict(logging=0, epsilon=EPS, dt=DT, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL,
                   mem_mode=L.SF_MEM_RESIDENT)
@@ -407,6 +413,51 @@ def bench_bh(steps: int = 3, pp_compare: bool = True) -> dict:
     out["pp_force_err_vs_fp64"] = pp_err
     for name in BH_RUNS:
         out[name]["pp_force_over_bh_step"] = pp_ms / out[name]["step_ms"]
+    return out
+
+
+def bench_bh_sweep(steps: int = 3) -> dict:
+    """The LBVH configurations at N = 4194304 (Plummer a = 1, seed 21, eps =
+    0.002): quadrupole cells at theta 0.55 to 0.75 and leaf 4, 6, 8, monopole
+    cells at theta 0.3 to 0.5 and leaf 3.  Per configuration: the force error
+    of sim_solv on the initial state against the FP64 direct sum (every 1024th
+    particle), then the mean time of `steps` KDK steps after one untimed one."""
+    N = 4194304
+    common = dict(logging=0, epsilon=EPS, dt=DT, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL,
+                  mem_mode=L.SF_MEM_RESIDENT)
+    s = Simulation(N, enable_tree=0, **common)
+    s.ic_plummer(seed=21, first=0, count=N)
+    pos, vel, mas = s.pos.copy(), s.vel.copy(), s.mas.copy()
+    idx = np.arange(0, N, 1024, dtype=np.uint32)
+    ref, _ = s.direct_f64(idx)
+    s.close()
+    configs = [dict(theta=t, leaf_size=lf, quadrupole=1) for t in (0.55, 0.6, 0.65, 0.7, 0.75) for lf in (4, 6, 8)]
+    configs += [dict(theta=t, leaf_size=3, quadrupole=0) for t in (0.3, 0.35, 0.4, 0.5)]
+    out = {"workload": f"Plummer sphere N={N} (a=1, M=1, seed 21), eps={EPS}, dt={DT}, fast policy, state resident; "
+                       "error of the initial solve vs the FP64 direct sum on every 1024th particle", "runs": []}
+    for cfg in configs:
+        s = Simulation(N, enable_tree=1, tree_mode=L.SF_TREE_LBVH, **cfg, **common)
+        s.pos[:] = pos
+        s.vel[:] = vel
+        s.mas[:] = mas
+        s.to_device()
+        s.solve()
+        s.to_host()
+        err = rel_err_pcts(s.acc[idx].astype(np.float64), ref)
+        s.step()
+        hip_sync()
+        s.lib.sf_prof_enable(1)
+        s.lib.sf_prof_reset()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.step()
+        hip_sync()
+        step_ms = (time.perf_counter() - t0) / steps * 1e3
+        w_ms, w_n = kernel_ms(s.lib, "lbvh_walk")
+        s.lib.sf_prof_enable(0)
+        s.close()
+        out["runs"].append({**cfg, "step_ms": step_ms, "walk_ms": w_ms / max(w_n, 1), "force_err_vs_fp64": err})
+    L.set_params()
     return out
 
 
@@ -713,8 +764,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the BH, adjoint and SPH measurements")
-    ap.add_argument("--only", choices=["bh", "adjoint", "adjoint64", "sph", "sphgrid", "copy", "bh1m", "stress",
-                                       "reftree", "vis", "run", "mirror"],
+    ap.add_argument("--only", choices=["bh", "bhsweep", "adjoint", "adjoint64", "sph", "sphgrid", "copy", "bh1m",
+                                       "stress", "reftree", "vis", "run", "mirror"],
                     help="run one secondary workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -725,7 +776,7 @@ def main():
         fn = {"bh": lambda: bench_bh(steps=args.steps, pp_compare=False), "adjoint": lambda: {"ckpt20": bench_adjoint(reps=1), "ckpt1": bench_adjoint(reps=1, ckpt=1)},
               "sph": lambda: bench_sph(steps=args.steps),
               "sphgrid": lambda: bench_sph(steps=args.steps, search=L.SF_SPH_SEARCH_GRID), "copy": bench_copy,
-              "bh1m": bench_bh1m, "adjoint64": bench_adjoint_fp64,
+              "bh1m": bench_bh1m, "adjoint64": bench_adjoint_fp64, "bhsweep": bench_bh_sweep,
               "stress": bench_stress, "reftree": bench_reftree, "vis": bench_vis, "run": bench_run,
               "mirror": bench_pp_mirror}[args.only]
         print(json.dumps(fn()))
