@@ -146,16 +146,19 @@ def test_degenerate_distributions_equal_direct_sum_when_every_node_opens(oracle,
     N = 3000
     rng = np.random.default_rng(77 + case)
     name, pos = degenerate_cases(N, rng)[case]
-    with lbvh_sim(N, theta=1e-6) as s:
+    # theta 1e-12: small nodes are sized by their particles' box, and on the
+    # line two neighbours 1e-7 apart make a node that theta 1e-6 would let
+    # groups 0.1 away accept (harmlessly: a 1e-12 relative monopole error)
+    with lbvh_sim(N, theta=1e-12) as s:
         s.pos[:] = pos.astype(np.float32)
         s.mas[:] = rng.uniform(0.5, 1.5, N).astype(np.float32) / N
         s.solve()
         ex = s.lbvh_export()
         assert ex["bad"] == 0, name
         if name in ("line", "grid"):
-            # elsewhere nodes of equal Morton keys have the finest cell size (coincident
-            # particles; 0.005-wide cells 1e4 from the other cluster), which distant groups
-            # rightly accept even at theta = 1e-6: the force check below still holds
+            # elsewhere nodes of coincident particles have a box of size zero, and the
+            # 0.005-wide clusters 1e4 apart tiny ones, which distant groups rightly accept
+            # even at theta = 1e-12: the force check below still holds
             assert ex["cells"] == 0 and ex["parts"] == N * N, name
         keys = oracle.morton63(s.pos.copy())
         assert np.array_equal(ex["keys"], np.sort(keys)), name
@@ -171,8 +174,11 @@ def test_degenerate_distributions_at_theta_half(oracle, case):
     """The accepted-cell path on the same distributions: finite forces, no
     malformed-tree report, and the monopole truncation error of a normal
     walk.  The error is measured against the RMS force, since on a line
-    the net force nearly cancels for many particles (measured: median
-    |da| / rms|a| 6.5e-3 on the line, below 1.5e-3 elsewhere)."""
+    the net force nearly cancels for many particles.  On the line a node's
+    particle box is a segment, whose monopole error at the same side /
+    distance is larger than a filled cube's (its traceless quadrupole does
+    not vanish): median |da| / rms|a| 1.5e-2 there (6.5e-3 when nodes were
+    sized by their Morton cube), below 1.5e-3 elsewhere."""
     N = 20000
     rng = np.random.default_rng(91 + case)
     name, pos = degenerate_cases(N, rng)[case]
@@ -186,7 +192,7 @@ def test_degenerate_distributions_at_theta_half(oracle, case):
         a64, _ = oracle.pp_f64(s.pos.copy(), s.mas.copy(), 1e-2, acc_phys=1, idx=idx.astype(np.uint32))
         d = np.linalg.norm(s.acc[idx] - a64, axis=1)
         rms = np.sqrt((np.linalg.norm(a64, axis=1) ** 2).mean())
-        assert np.median(d) / rms < 1e-2, (name, np.median(d) / rms)
+        assert np.median(d) / rms < (2e-2 if name == "line" else 1e-2), (name, np.median(d) / rms)
         if name != "line":
             assert np.median(rel_err(s.acc[idx], a64)) < 5e-3, name
     L.set_params()
