@@ -343,21 +343,25 @@ def rel_err_pcts(acc, ref):
 BH_RUNS = {
     # monopole cells: the reference tree's expansion, at the headline theta
     "monopole_theta0.5": dict(theta=0.5, leaf_size=3, quadrupole=0),
-    # the fastest configuration of a sweep at this N (quadrupole theta 0.55 to
-    # 0.75 at leaf 4, 6, 8; monopole theta 0.3 to 0.5) whose 99th-percentile
-    # force error stays within 1e-3: quadrupole cells, theta 0.6, leaf 4
+    # the fastest configuration of the sweeps at this N whose 99th-percentile
+    # force error stays within 1e-3: quadrupole cells under the relative
+    # opening test, G M l^3 <= alpha |a| d^5 with alpha 3e-3; theta 1.15 is the
+    # largest that keeps every accepted cell's particles outside the group's
+    # box (1 / theta >= sqrt(3) / 2)
+    "quadrupole_relative": dict(theta=1.15, leaf_size=6, quadrupole=1, mac_alpha=3e-3),
+    # the fastest with the theta test alone: quadrupole cells, theta 0.6, leaf 4
     "quadrupole_theta0.6": dict(theta=0.6, leaf_size=4, quadrupole=1),
-    # and with more margin: theta 0.55
-    "quadrupole_theta0.55": dict(theta=0.55, leaf_size=4, quadrupole=1),
+    # monopole cells under the relative test (G M l^2 <= alpha |a| d^4),
+    # alpha 1e-3: faster than theta 0.5 alone, with a smaller 99th percentile
+    "monopole_relative": dict(theta=1.15, leaf_size=3, quadrupole=0, mac_alpha=1e-3),
 }
 
 
 def bench_bh(steps: int = 3, pp_compare: bool = True) -> dict:
     """Barnes-Hut (SF_TREE_LBVH) KDK steps at N = 4194304 (Plummer a = 1,
-    seeded): monopole cells at theta = 0.5, and quadrupole cells at theta =
-    0.6 and 0.55 (leaf 4), against one fast PP force evaluation of the same
-    state; force errors of all against the FP64 direct sum on every 1024th
-    particle of each run's final state."""
+    seeded) in the configurations of BH_RUNS, against one fast PP force
+    evaluation of the same state; force errors of all against the FP64
+    direct sum on every 1024th particle of each run's final state."""
     N = 4194304
     common = dict(logging=0, epsilon=EPS, dt=DT, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL,
                   mem_mode=L.SF_MEM_RESIDENT)
@@ -416,12 +420,24 @@ def bench_bh(steps: int = 3, pp_compare: bool = True) -> dict:
     return out
 
 
-def bench_bh_sweep(steps: int = 3) -> dict:
-    """The LBVH configurations at N = 4194304 (Plummer a = 1, seed 21, eps =
-    0.002): quadrupole cells at theta 0.55 to 0.75 and leaf 4, 6, 8, monopole
-    cells at theta 0.3 to 0.5 and leaf 3.  Per configuration: the force error
-    of sim_solv on the initial state against the FP64 direct sum (every 1024th
-    particle), then the mean time of `steps` KDK steps after one untimed one."""
+SWEEP_THETA = [dict(theta=t, leaf_size=lf, quadrupole=1) for t in (0.55, 0.6, 0.65, 0.7, 0.75) for lf in (4, 6, 8)] + \
+              [dict(theta=t, leaf_size=3, quadrupole=0) for t in (0.3, 0.35, 0.4, 0.5)]
+# the relative opening test: theta is the safety bound, mac_alpha the error target
+SWEEP_ALPHA = [dict(theta=t, leaf_size=3, quadrupole=0, mac_alpha=a) for t in (1.0, 1.15)
+               for a in (5e-4, 7e-4, 1e-3, 1.5e-3)] + \
+              [dict(theta=t, leaf_size=lf, quadrupole=1, mac_alpha=a) for t in (1.0, 1.15) for lf in (4, 6, 8)
+               for a in (2e-3, 3e-3, 4e-3)]
+
+
+def bench_bh_sweep(steps: int = 3, configs=None) -> dict:
+    """LBVH configurations at N = 4194304 (Plummer a = 1, seed 21, eps =
+    0.002), by default quadrupole cells at theta 0.55 to 0.75 and leaf 4, 6, 8,
+    monopole cells at theta 0.3 to 0.5 and leaf 3.  Per configuration: the
+    force error of sim_solv on the initial state against the FP64 direct sum
+    (every 1024th particle) and the walk's interactions per particle, the
+    mean time of `steps` KDK steps after one untimed one, and the force error
+    of the state after them."""
+    configs = SWEEP_THETA if configs is None else configs
     N = 4194304
     common = dict(logging=0, epsilon=EPS, dt=DT, precision=L.SF_PRECISION_FAST, acc_mode=L.SF_ACC_PHYSICAL,
                   mem_mode=L.SF_MEM_RESIDENT)
@@ -431,8 +447,6 @@ def bench_bh_sweep(steps: int = 3) -> dict:
     idx = np.arange(0, N, 1024, dtype=np.uint32)
     ref, _ = s.direct_f64(idx)
     s.close()
-    configs = [dict(theta=t, leaf_size=lf, quadrupole=1) for t in (0.55, 0.6, 0.65, 0.7, 0.75) for lf in (4, 6, 8)]
-    configs += [dict(theta=t, leaf_size=3, quadrupole=0) for t in (0.3, 0.35, 0.4, 0.5)]
     out = {"workload": f"Plummer sphere N={N} (a=1, M=1, seed 21), eps={EPS}, dt={DT}, fast policy, state resident; "
                        "error of the initial solve vs the FP64 direct sum on every 1024th particle", "runs": []}
     for cfg in configs:
@@ -444,6 +458,8 @@ def bench_bh_sweep(steps: int = 3) -> dict:
         s.solve()
         s.to_host()
         err = rel_err_pcts(s.acc[idx].astype(np.float64), ref)
+        ex = s.lbvh_export()
+        inter = (ex["cells"] + ex["parts"]) / N
         s.step()
         hip_sync()
         s.lib.sf_prof_enable(1)
@@ -455,8 +471,13 @@ def bench_bh_sweep(steps: int = 3) -> dict:
         step_ms = (time.perf_counter() - t0) / steps * 1e3
         w_ms, w_n = kernel_ms(s.lib, "lbvh_walk")
         s.lib.sf_prof_enable(0)
+        s.to_host()
+        ref_end, _ = s.direct_f64(idx)
+        err_end = rel_err_pcts(s.acc[idx].astype(np.float64), ref_end)
         s.close()
-        out["runs"].append({**cfg, "step_ms": step_ms, "walk_ms": w_ms / max(w_n, 1), "force_err_vs_fp64": err})
+        out["runs"].append({**cfg, "step_ms": step_ms, "walk_ms": w_ms / max(w_n, 1),
+                            "interactions_per_particle": inter, "force_err_vs_fp64": err,
+                            "force_err_after_steps": err_end})
     L.set_params()
     return out
 
@@ -765,7 +786,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the BH, adjoint and SPH measurements")
     ap.add_argument("--only", choices=["bh", "bhsweep", "adjoint", "adjoint64", "sph", "sphgrid", "copy", "bh1m",
-                                       "stress", "reftree", "vis", "run", "mirror"],
+                                       "stress", "reftree", "vis", "run", "mirror", "macsweep"],
                     help="run one secondary workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -778,7 +799,8 @@ def main():
               "sphgrid": lambda: bench_sph(steps=args.steps, search=L.SF_SPH_SEARCH_GRID), "copy": bench_copy,
               "bh1m": bench_bh1m, "adjoint64": bench_adjoint_fp64, "bhsweep": bench_bh_sweep,
               "stress": bench_stress, "reftree": bench_reftree, "vis": bench_vis, "run": bench_run,
-              "mirror": bench_pp_mirror}[args.only]
+              "mirror": bench_pp_mirror,
+              "macsweep": lambda: bench_bh_sweep(configs=SWEEP_ALPHA)}[args.only]
         print(json.dumps(fn()))
         return
 

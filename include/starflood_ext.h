@@ -114,6 +114,18 @@ typedef struct {
 	                                as well as monopoles (0: monopoles, as the
 	                                reference's tree) */
 	int sph_search;              /* enum sf_sph_search of the cubic model (GRID) */
+	double mac_alpha;            /* SF_TREE_LBVH relative opening test (0: off, the
+	                                default): a cell that passes the theta test is
+	                                also required to satisfy G M l^2 <= mac_alpha
+	                                |a|_min d^4 (quadrupole cells: G M l^3 <=
+	                                mac_alpha |a|_min d^5), with l its size, d its
+	                                distance to the walk group's box and |a|_min the
+	                                smallest
+	                                previous acceleration in the group (sim_step
+	                                reads it from the acc block; sim_solv and
+	                                solve_gravity_part_tree first walk with the theta
+	                                test alone).  Bounds the force error relative to
+	                                each particle's acceleration. */
 } sf_params_t;
 
 /* Fill *p with the defaults listed above. */

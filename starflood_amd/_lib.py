@@ -57,7 +57,8 @@ class Params(C.Structure):
                 ("kahan_energy", C.c_int), ("precision", C.c_int), ("acc_mode", C.c_int),
                 ("tree_mode", C.c_int), ("sph_model", C.c_int), ("mem_mode", C.c_int),
                 ("sph_gamma", C.c_double), ("logging", C.c_int), ("verbose", C.c_int),
-                ("out_dir", C.c_char * 256), ("quadrupole", C.c_int), ("sph_search", C.c_int)]
+                ("out_dir", C.c_char * 256), ("quadrupole", C.c_int), ("sph_search", C.c_int),
+                ("mac_alpha", C.c_double)]
 
 
 class VisT(C.Structure):

@@ -150,7 +150,8 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
                                                  int N, const unsigned* __restrict__ group_start,
                                                  const float4* __restrict__ rec_a,
                                                  const float4* __restrict__ node_q0, const float* __restrict__ node_q1,
-                                                 const int4* __restrict__ rec_b, float eps2, double G, int acc_phys,
+                                                 const int4* __restrict__ rec_b, const float* __restrict__ node_q,
+                                                 float alpha_g, float eps2, double G, int acc_phys,
                                                  float* __restrict__ acc, float* __restrict__ pot,
                                                  unsigned long long* __restrict__ counters, int* __restrict__ gstack,
                                                  unsigned n_chunks) {
@@ -202,6 +203,21 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
 			const float xi = me.x, yi = me.y, zi = me.z;
 			const float gx0 = wave_min(xi), gy0 = wave_min(yi), gz0 = wave_min(zi);
 			const float gx1 = wave_max(xi), gy1 = wave_max(yi), gz1 = wave_max(zi);
+			// relative opening test: am = alpha / G times the smallest previous
+			// |a| of the group (0: theta test alone; also when no particle has a
+			// positive finite previous acceleration)
+			float am = 0.0f;
+			if (alpha_g > 0.0f) {
+				float amag = 3.0e38f;
+				if (live) {
+					const unsigned i0 = order[p];
+					const float ax = acc[3u * i0], ay = acc[3u * i0 + 1u], az = acc[3u * i0 + 2u];
+					const float a = sqrtf(ax * ax + ay * ay + az * az);
+					amag = acc_phys ? a : a / me.w;   // the reference's acc is m a
+				}
+				amag = wave_min(amag);   // fminf drops NaN
+				am = (amag > 0.0f && amag < 3.0e38f) ? alpha_g * amag : 0.0f;
+			}
 
 			Acc a;
 			a.tx = a.ty = a.tz = a.tp = a.sx = a.sy = a.sz = a.sp = a.cx = a.cy = a.cz = a.cp = 0.0f;
@@ -234,7 +250,13 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
 						const float dy = fmaxf(fmaxf(gy0 - ra.y, ra.y - gy1), 0.0f);
 						const float dz = fmaxf(fmaxf(gz0 - ra.z, ra.z - gz1), 0.0f);
 						const float d2 = dx * dx + dy * dy + dz * dz;
-						if (d2 > __int_as_float(rb.x) && abs(rb.w) > 1) cls = 1;
+						// relative test: G M l^p <= alpha |a| d^(p+2), p = 2 (monopole) or 3
+						bool rel = true;
+						if (am > 0.0f) {
+							const float d4 = d2 * d2;
+							rel = node_q[node] <= am * (QUAD ? d4 * sqrtf(d2) : d4);
+						}
+						if (d2 > __int_as_float(rb.x) && abs(rb.w) > 1 && rel) cls = 1;
 						else if (rb.w > 0) cls = 2;   // (first, count) record
 						else cls = 3;
 					}
@@ -404,9 +426,9 @@ extern "C" {
 size_t sfd_lbvh_walk_stack_bytes(void) { return (size_t)resident_blocks() * kWaves * kStack * sizeof(int); }
 
 int sfd_lbvh_walk(const void* body, const unsigned* order, unsigned N, const unsigned* group_start,
-                  const void* node_cm, const void* node_q0, const float* node_q1, const void* rec_b, float eps2,
-                  double G, int acc_phys, float* acc, float* pot, unsigned long long* counters, int* gstack,
-                  hipStream_t s) {
+                  const void* node_cm, const void* node_q0, const float* node_q1, const void* rec_b,
+                  const float* node_q, float alpha_g, float eps2, double G, int acc_phys, float* acc, float* pot,
+                  unsigned long long* counters, int* gstack, hipStream_t s) {
 	if (N == 0) return 0;
 	if (!counters || !gstack || !group_start) return -1;
 	const bool quad = node_q0 && node_q1;
@@ -419,13 +441,13 @@ int sfd_lbvh_walk(const void* body, const unsigned* order, unsigned N, const uns
 	if (quad)
 		hipLaunchKernelGGL(k_walk<true>, dim3(blocks), dim3(kBlock), 0, s, static_cast<const float4*>(body), order,
 		                   (int)N, group_start, static_cast<const float4*>(node_cm),
-		                   static_cast<const float4*>(node_q0), node_q1, static_cast<const int4*>(rec_b), eps2, G,
-		                   acc_phys, acc, pot, counters, gstack, chunks);
+		                   static_cast<const float4*>(node_q0), node_q1, static_cast<const int4*>(rec_b), node_q,
+		                   alpha_g, eps2, G, acc_phys, acc, pot, counters, gstack, chunks);
 	else
 		hipLaunchKernelGGL(k_walk<false>, dim3(blocks), dim3(kBlock), 0, s, static_cast<const float4*>(body), order,
 		                   (int)N, group_start, static_cast<const float4*>(node_cm), (const float4*)nullptr,
-		                   (const float*)nullptr, static_cast<const int4*>(rec_b), eps2, G, acc_phys, acc, pot,
-		                   counters, gstack, chunks);
+		                   (const float*)nullptr, static_cast<const int4*>(rec_b), node_q, alpha_g, eps2, G, acc_phys,
+		                   acc, pot, counters, gstack, chunks);
 	sf_prof_end(tok, s);
 	return sf_hip_check(hipGetLastError(), "sfd_lbvh_walk", "launch");
 }

@@ -39,6 +39,7 @@ void sf_params_default(sf_params_t* p) {
 	strcpy(p->out_dir, "./out");    /* config.h:75 */
 	p->quadrupole = 0;
 	p->sph_search = SF_SPH_SEARCH_GRID;
+	p->mac_alpha = 0.0;
 }
 
 const sf_params_t* sf_params(void) {
@@ -59,6 +60,10 @@ int sf_set_params(const sf_params_t* p) {
 	}
 	if (!(p->epsilon >= 0.0) || !(p->dt == p->dt) || !(p->theta >= 0.0)) {
 		sf_error("sf_set_params", "epsilon and theta must be >= 0 and dt a number");
+		return STARFLOOD_FAILURE;
+	}
+	if (!(p->mac_alpha >= 0.0) || p->mac_alpha > 1.0e30) {
+		sf_error("sf_set_params", "mac_alpha must be finite and >= 0");
 		return STARFLOOD_FAILURE;
 	}
 	if (p->tree_depth_max < 1u || p->tree_depth_max > 16u) {

@@ -19,8 +19,10 @@ unsigned int sf_pp_slice_count(unsigned int n_pad);
 size_t sf_pp_part_bytes(unsigned int n_pad);
 int sf_pp_force_arrays(const sf_params_t* p, unsigned int N, unsigned int n_pad, const float* pos, const float* mas,
                        float* body, float* part, size_t part_bytes, float* acc, float* pot, hipStream_t s);
-/* tree.c: bounds as src/simulation.c:245-301, build, then walk into acc/pot */
-int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number);
+/* tree.c: bounds as src/simulation.c:245-301, build, then walk into acc/pot.
+   prev_acc: acc holds the previous force evaluation (sim_step), which the
+   LBVH relative opening test reads; otherwise it first walks without it. */
+int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number, int prev_acc);
 /* tree.c: after an LBVH solve on stream s, fail if the walk met a malformed
    tree (synchronises s; no-op in reference mode) */
 int sf_tree_check(tree_t* tree, hipStream_t s, const char* fn);
@@ -35,13 +37,13 @@ int sf_sim_solv(sim_t* SF_RESTRICT sim_ptr);
 int sf_sim_step(sim_t* SF_RESTRICT sim_ptr);
 /* simulation.c: the solver sequence of sim_solv on device state
    (sph_pred_dt: see sf_sph_prepare) */
-int sf_solve_all(sf_ctx* c, tree_t* tree, unsigned int step_number, float sph_pred_dt);
+int sf_solve_all(sf_ctx* c, tree_t* tree, unsigned int step_number, float sph_pred_dt, int prev_acc);
 
 /* lbvh.c: SF_TREE_LBVH build (allocates *st on first use) and walk */
 typedef struct sf_lbvh sf_lbvh;
 int sf_lbvh_build(sf_lbvh** st, unsigned int N, const float* pos, const float* mas, int quadrupole, hipStream_t s);
-int sf_lbvh_walk(sf_lbvh* st, unsigned int N, const float* pos, const float* mas, const sf_params_t* p, float* acc,
-                 float* pot, hipStream_t s);
+int sf_lbvh_walk(sf_lbvh* st, unsigned int N, const float* pos, const float* mas, const sf_params_t* p, int prev_acc,
+                 float* acc, float* pot, hipStream_t s);
 void sf_lbvh_destroy(sf_lbvh* st);
 /* particles of the LBVH last built (0: none, or no build yet) */
 unsigned int sf_lbvh_size(const sf_lbvh* st);

@@ -164,13 +164,13 @@ static int download_solver_outputs(sf_ctx* c) {
 }
 
 /* Solver sequence of src/simulation.c:231-345 on device state. */
-int sf_solve_all(sf_ctx* c, tree_t* tree, unsigned int step_number, float sph_pred_dt) {
+int sf_solve_all(sf_ctx* c, tree_t* tree, unsigned int step_number, float sph_pred_dt, int prev_acc) {
 	const sf_params_t* p = sf_params();
 	int rc = 0;
 	if (c->N == 0) return 0;   /* no particles: no forces, no densities */
 	if (p->enable_sph && sf_sph_prepare(c, sph_pred_dt)) return -1;
 	if (p->enable_tree) {
-		if (sf_solve_gravity_tree(c, tree, step_number)) rc = -1;
+		if (sf_solve_gravity_tree(c, tree, step_number, prev_acc)) rc = -1;
 	} else {
 		if (sf_solve_gravity_pp(c)) rc = -1;
 	}
@@ -188,7 +188,7 @@ int sf_sim_solv(sim_t* SF_RESTRICT sim_ptr) {
 	if (!c) return STARFLOOD_FAILURE;
 	int rc = 0;
 	if (upload_solver_inputs(c)) return STARFLOOD_FAILURE;
-	if (sf_solve_all(c, &sim_ptr->tree, sim_ptr->step_number, 0.0f)) rc = -1;
+	if (sf_solve_all(c, &sim_ptr->tree, sim_ptr->step_number, 0.0f, 0)) rc = -1;
 	if (download_solver_outputs(c)) rc = -1;
 	if (sf_finish(c, "sim_solv")) rc = -1;
 	if (sf_params()->enable_tree && c->N && sf_tree_check(&sim_ptr->tree, c->stream, "sim_solv")) rc = -1;
@@ -248,7 +248,8 @@ int sf_sim_step(sim_t* SF_RESTRICT sim_ptr) {
 	}
 	if (timing) tev_record(2, s);
 
-	if (sf_solve_all(c, &sim_ptr->tree, sim.step_number, cubic ? half_dt : 0.0f)) {
+	/* acc still holds the force that kick_0 used: the previous evaluation */
+	if (sf_solve_all(c, &sim_ptr->tree, sim.step_number, cubic ? half_dt : 0.0f, 1)) {
 		fprintf(stderr, "%s error: %s failed.\n", "sim_step()", "sim_solv()");
 		rc = -1;
 	}

@@ -266,7 +266,7 @@ static void bounds_jitter(unsigned int step_number, double jitter[4]) {
 }
 
 /* The tree part of src/simulation.c:245-322 on device state. */
-int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number) {
+int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number, int prev_acc) {
 	sf_tree_dev* d = tree_dev_lookup(tree, "solve_gravity_part_tree");
 	if (!d) return -1;
 	const sf_params_t* p = sf_params();
@@ -302,7 +302,10 @@ int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number) {
 	}
 	if (sf_lbvh_build(&d->lbvh, N, pos, mas, p->quadrupole, s)) return -1;
 	tree->num_nodes = N ? 2u * N - 1u : 0u;
-	return sf_lbvh_walk(d->lbvh, N, pos, mas, p, acc, pot, s);
+	/* the relative opening test needs every particle's previous acceleration:
+	   without one, a walk with the theta test alone provides it */
+	if (p->mac_alpha > 0.0 && !prev_acc && sf_lbvh_walk(d->lbvh, N, pos, mas, p, 0, acc, pot, s)) return -1;
+	return sf_lbvh_walk(d->lbvh, N, pos, mas, p, p->mac_alpha > 0.0, acc, pot, s);
 }
 
 int sf_lbvh_export(const tree_t* tree_ptr, unsigned long long* keys, unsigned int* order,
@@ -354,7 +357,10 @@ static int gravity_part_tree(const sim_t* SF_RESTRICT sim_ptr) {
 	if (p->tree_mode == SF_TREE_REFERENCE) {
 		rc = walk_ref(d, N, pos, mas, (float)(p->epsilon * p->epsilon), acc, pot, s);
 	} else {
-		rc = d->lbvh ? sf_lbvh_walk(d->lbvh, N, pos, mas, p, acc, pot, s) : -1;
+		/* standalone: acc holds no previous force, so the relative opening
+		   test (mac_alpha > 0) starts from a theta-only walk */
+		rc = d->lbvh ? sf_lbvh_walk(d->lbvh, N, pos, mas, p, 0, acc, pot, s) : -1;
+		if (!rc && d->lbvh && p->mac_alpha > 0.0) rc = sf_lbvh_walk(d->lbvh, N, pos, mas, p, 1, acc, pot, s);
 		if (!d->lbvh) sf_error("solve_gravity_part_tree", "no tree was built");
 		if (!rc && lbvh_check(d, s, "solve_gravity_part_tree")) rc = -1;
 	}

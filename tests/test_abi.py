@@ -125,6 +125,15 @@ def test_set_params_rejects_bad_values():
     p = L.default_params()
     p.precision = 7
     assert lib.sf_set_params(C.byref(p)) == L.STARFLOOD_FAILURE
+    for bad in (-1e-3, float("nan"), float("inf")):
+        p = L.default_params()
+        p.mac_alpha = bad
+        assert lib.sf_set_params(C.byref(p)) == L.STARFLOOD_FAILURE
+        assert "mac_alpha" in L.last_error()
+    p = L.default_params()
+    assert p.mac_alpha == 0.0                       # off: the theta test alone, as the reference
+    p.mac_alpha = 3e-3
+    assert lib.sf_set_params(C.byref(p)) == L.STARFLOOD_SUCCESS
     L.set_params()
 
 

@@ -198,6 +198,81 @@ def test_degenerate_distributions_at_theta_half(oracle, case):
     L.set_params()
 
 
+def _relative_run(N, theta, alpha, steps, acc_mode=L.SF_ACC_PHYSICAL, quadrupole=0):
+    """sim_solv then `steps` KDK steps on a Plummer sphere (seed 21); the
+    final acc, the FP64 reference on every 32nd particle and the walk's
+    interaction count per particle."""
+    with lbvh_sim(N, theta=theta, mac_alpha=alpha, epsilon=5e-3, acc_mode=acc_mode, quadrupole=quadrupole,
+                  leaf_size=4 if quadrupole else 3, mem_mode=L.SF_MEM_RESIDENT) as s:
+        s.ic_plummer(seed=21, first=0, count=N)
+        s.solve()
+        for n in range(steps):
+            s.step_number = n
+            s.step()
+        s.to_host()
+        ex = s.lbvh_export()
+        assert ex["bad"] == 0
+        idx = np.arange(0, N, 32, dtype=np.uint32)
+        a64, _ = s.direct_f64(idx)
+        return s.acc[idx].copy(), a64, (ex["cells"] + ex["parts"]) / N, s.mas[idx].copy()
+
+
+def test_relative_opening_test_bounds_the_error_relative_to_each_acceleration(oracle):
+    """mac_alpha > 0 also requires G M l^2 <= alpha |a|_min d^4 of every cell,
+    with |a|_min from the group's previous accelerations.  At N = 131072:
+    - it only removes cells, so at the same theta it evaluates more
+      interactions and every error percentile is lower;
+    - at theta 1 and alpha 2e-4 the 99th percentile stays within 2e-3, and the
+      errors spread less (99th percentile over median) than with the theta
+      test alone at theta 0.5;
+    - alpha 1e30 lets every cell pass, and the forces equal the theta walk's
+      bit for bit;
+    - it holds after two KDK steps and in the reference's acc = m a mode."""
+    N = 131072
+    acc_t, a64t, n_t, _ = _relative_run(N, 0.7, 0.0, 2)
+    acc_r, a64r, n_r, _ = _relative_run(N, 0.7, 2e-4, 2)
+    e_t, e_r = rel_err(acc_t, a64t), rel_err(acc_r, a64r)
+    print(f"theta 0.7: {n_t:.0f} interactions/particle, p50 {np.percentile(e_t, 50):.3g}, p99 "
+          f"{np.percentile(e_t, 99):.3g}; with alpha 2e-4: {n_r:.0f}, p50 {np.percentile(e_r, 50):.3g}, "
+          f"p99 {np.percentile(e_r, 99):.3g}")
+    assert n_r > n_t
+    for q in (50, 90, 99):
+        assert np.percentile(e_r, q) < np.percentile(e_t, q)
+    acc_1, a64, n_1, _ = _relative_run(N, 1.0, 2e-4, 2)
+    acc_5, a645, n_5, _ = _relative_run(N, 0.5, 0.0, 2)
+    e_1, e_5 = rel_err(acc_1, a64), rel_err(acc_5, a645)
+    print(f"theta 1, alpha 2e-4: {n_1:.0f} interactions/particle, p50 {np.percentile(e_1, 50):.3g}, p99 "
+          f"{np.percentile(e_1, 99):.3g}; theta 0.5 alone: {n_5:.0f}, p50 {np.percentile(e_5, 50):.3g}, p99 "
+          f"{np.percentile(e_5, 99):.3g}")
+    assert np.percentile(e_1, 99) <= 2e-3
+    assert np.percentile(e_1, 99) / np.percentile(e_1, 50) < np.percentile(e_5, 99) / np.percentile(e_5, 50)
+    acc_h, _, _, _ = _relative_run(N, 0.7, 1e30, 2)
+    assert np.array_equal(acc_h.view(np.uint32), acc_t.view(np.uint32))
+    acc_m, a64m, _, m = _relative_run(N, 1.0, 2e-4, 2, acc_mode=L.SF_ACC_REFERENCE)
+    e_m = rel_err(acc_m, a64m)
+    assert np.percentile(e_m, 99) <= 2e-3
+    L.set_params()
+
+
+def test_relative_opening_test_with_no_cells_equals_the_direct_sum(oracle):
+    """alpha 1e-30 rejects every cell: only leaf particles are summed, and the
+    forces equal the FP64 direct sum to FP32 rounding (solve and one step)."""
+    N = 4096
+    rng = np.random.default_rng(5)
+    with lbvh_sim(N, theta=0.5, mac_alpha=1e-30) as s:
+        s.pos[:] = rng.normal(size=(N, 3)).astype(np.float32)
+        s.mas[:] = rng.uniform(0.5, 1.5, N).astype(np.float32) / N
+        s.solve()
+        s.step_number = 0
+        s.step()
+        ex = s.lbvh_export()
+        assert ex["bad"] == 0 and ex["cells"] == 0 and ex["parts"] == N * N
+        a64, _ = oracle.pp_f64(s.pos.copy(), s.mas.copy(), 1e-2, acc_phys=1)
+        scale = np.linalg.norm(a64, axis=1).max()
+        assert np.abs(s.acc - a64).max() <= 2e-6 * scale
+    L.set_params()
+
+
 def test_mirror_and_resident_modes_give_identical_forces():
     N = 30000
     outs = []
