@@ -394,7 +394,8 @@ def bench_bh(steps: int = 3, pp_compare: bool = True) -> dict:
         walk_ms = w_ms / max(w_n, 1)
         inter = float(ex["cells"] + ex["parts"])
         out[name] = {**cfg, "step_ms": step_ms, "build_ms": b_ms / max(b_n, 1), "walk_ms": walk_ms,
-                     "interactions_per_particle": inter / N, "walk_interactions_per_s": inter / (walk_ms / 1e3),
+                     "interactions_per_particle": inter / N, "cell_interactions_per_particle": ex["cells"] / N,
+                     "walk_interactions_per_s": inter / (walk_ms / 1e3),
                      "force_err_vs_fp64": rel_err_pcts(s.acc[idx].astype(np.float64), run_ref)}
         s.close()
         L.set_params()
