@@ -150,8 +150,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
                                                  int N, const unsigned* __restrict__ group_start,
                                                  const float4* __restrict__ rec_a,
                                                  const float4* __restrict__ node_q0, const float* __restrict__ node_q1,
-                                                 const int4* __restrict__ rec_b, const float* __restrict__ node_q,
-                                                 float alpha_g, float eps2, double G, int acc_phys,
+                                                 const int4* __restrict__ rec_b, float alpha_g, float eps2, double G, int acc_phys,
                                                  float* __restrict__ acc, float* __restrict__ pot,
                                                  unsigned long long* __restrict__ counters, int* __restrict__ gstack,
                                                  unsigned n_chunks) {
@@ -250,14 +249,17 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
 						const float dy = fmaxf(fmaxf(gy0 - ra.y, ra.y - gy1), 0.0f);
 						const float dz = fmaxf(fmaxf(gz0 - ra.z, ra.z - gz1), 0.0f);
 						const float d2 = dx * dx + dy * dy + dz * dz;
+						// the sign bit of the r_crit^2 word marks a (first, count) record
+						const bool leafy = rb.x < 0;
 						// relative test: G M l^p <= alpha |a| d^(p+2), p = 2 (monopole) or 3
 						bool rel = true;
 						if (am > 0.0f) {
 							const float d4 = d2 * d2;
-							rel = node_q[node] <= am * (QUAD ? d4 * sqrtf(d2) : d4);
+							rel = __int_as_float(rb.w) <= am * (QUAD ? d4 * sqrtf(d2) : d4);
 						}
-						if (d2 > __int_as_float(rb.x) && abs(rb.w) > 1 && rel) cls = 1;
-						else if (rb.w > 0) cls = 2;   // (first, count) record
+						// a cell holds at least two particles (every internal node does)
+						if (d2 > __int_as_float(rb.x & 0x7fffffff) && (!leafy || rb.z > 1) && rel) cls = 1;
+						else if (leafy) cls = 2;
 						else cls = 3;
 					}
 					const unsigned long long m_cell = __ballot(cls == 1);
@@ -427,7 +429,7 @@ size_t sfd_lbvh_walk_stack_bytes(void) { return (size_t)resident_blocks() * kWav
 
 int sfd_lbvh_walk(const void* body, const unsigned* order, unsigned N, const unsigned* group_start,
                   const void* node_cm, const void* node_q0, const float* node_q1, const void* rec_b,
-                  const float* node_q, float alpha_g, float eps2, double G, int acc_phys, float* acc, float* pot,
+                  float alpha_g, float eps2, double G, int acc_phys, float* acc, float* pot,
                   unsigned long long* counters, int* gstack, hipStream_t s) {
 	if (N == 0) return 0;
 	if (!counters || !gstack || !group_start) return -1;
@@ -441,12 +443,12 @@ int sfd_lbvh_walk(const void* body, const unsigned* order, unsigned N, const uns
 	if (quad)
 		hipLaunchKernelGGL(k_walk<true>, dim3(blocks), dim3(kBlock), 0, s, static_cast<const float4*>(body), order,
 		                   (int)N, group_start, static_cast<const float4*>(node_cm),
-		                   static_cast<const float4*>(node_q0), node_q1, static_cast<const int4*>(rec_b), node_q,
+		                   static_cast<const float4*>(node_q0), node_q1, static_cast<const int4*>(rec_b),
 		                   alpha_g, eps2, G, acc_phys, acc, pot, counters, gstack, chunks);
 	else
 		hipLaunchKernelGGL(k_walk<false>, dim3(blocks), dim3(kBlock), 0, s, static_cast<const float4*>(body), order,
 		                   (int)N, group_start, static_cast<const float4*>(node_cm), (const float4*)nullptr,
-		                   (const float*)nullptr, static_cast<const int4*>(rec_b), node_q, alpha_g, eps2, G, acc_phys,
+		                   (const float*)nullptr, static_cast<const int4*>(rec_b), alpha_g, eps2, G, acc_phys,
 		                   acc, pot, counters, gstack, chunks);
 	sf_prof_end(tok, s);
 	return sf_hip_check(hipGetLastError(), "sfd_lbvh_walk", "launch");

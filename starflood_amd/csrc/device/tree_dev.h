@@ -95,19 +95,19 @@ int sfd_lbvh_topology(const float* pos, const float* mas, const unsigned long lo
                       unsigned int N, const double* box, void* body, void* node_lr, void* node_range, void* prefix,
                       void* tile_sum, void* node_cm, void* node_cell, hipStream_t s);
 int sfd_lbvh_records(unsigned int N, const void* node_lr, const void* node_range, const void* node_cm,
-                     const void* node_cell, float theta, int leaf_size, void* rec_b, float* node_q, int quadrupole,
-                     hipStream_t s);
+                     const void* node_cell, float theta, int leaf_size, int quadrupole, void* rec_b, hipStream_t s);
 /* lbvh_walk.hip.  counters (5 x u64; the first four zeroed here): cell
    interactions, particle interactions, wavefronts that met a malformed tree,
    node batches; the fifth adds the third up over walks until the host clears
    it.
    gstack: sfd_lbvh_walk_stack_bytes() of scratch.  alpha_g > 0 adds the
-   relative opening test M l^2 <= alpha_g |a|_min d^4 (node_q = M l^2 from
-   sfd_lbvh_records), |a|_min over each group's previous acc. */
+   relative opening test M l^p <= alpha_g |a|_min d^(p+2) (the records' size
+   term M l^p, p = 2, or 3 for quadrupole cells), |a|_min over each group's
+   previous acc. */
 size_t sfd_lbvh_walk_stack_bytes(void);
 int sfd_lbvh_walk(const void* body, const unsigned int* order, unsigned int N, const unsigned int* group_start,
                   const void* node_cm, const void* node_q0, const float* node_q1, const void* rec_b,
-                  const float* node_q, float alpha_g, float eps2, double G, int acc_phys, float* acc, float* pot,
+                  float alpha_g, float eps2, double G, int acc_phys, float* acc, float* pot,
                   unsigned long long* counters, int* gstack, hipStream_t s);
 /* traceless quadrupoles (node_q0 2N-1 float4, node_q1 2N-1 float) from a
    second FP64 prefix scan (prefix2: N x 6 doubles, tile_sum2:

@@ -31,7 +31,6 @@ struct sf_lbvh {
 	void* node_cm;            /* 2N-1 float4: com, mass */
 	void* node_cell;          /* 2N-1 float4: Morton cell centre, side */
 	void* rec_b;              /* 2N-1 int4 */
-	float* node_q;            /* 2N-1: M l^2 of the relative opening test */
 	unsigned long long* counters;   /* 5: the walk's four (zeroed per walk), then the sticky guard count */
 	unsigned long long* h_counters; /* pinned */
 	int* gstack;                    /* walk node stacks, one per wavefront slot */
@@ -47,7 +46,7 @@ struct sf_lbvh {
 
 static void free_buffers(sf_lbvh* t) {
 	void* ptrs[] = {t->key0, t->key1, t->val0, t->val1, t->hist, t->bbox_part, t->box, t->body, t->node_lr,
-	                t->node_range, t->prefix, t->tile_sum, t->node_cm, t->node_cell, t->rec_b, t->node_q, t->counters,
+	                t->node_range, t->prefix, t->tile_sum, t->node_cm, t->node_cell, t->rec_b, t->counters,
 	                t->gstack, t->group_start, t->prefix2, t->tile_sum2, t->quad_ref, t->node_q0, t->node_q1};
 	for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k)
 		if (ptrs[k]) hipFree(ptrs[k]);
@@ -96,7 +95,6 @@ static int ensure(sf_lbvh** st, unsigned int N) {
 	rc |= alloc(&t->node_cm, nn * 16u);
 	rc |= alloc(&t->node_cell, nn * 16u);
 	rc |= alloc(&t->rec_b, nn * 16u);
-	rc |= alloc((void**)&t->node_q, nn * 4u);
 	rc |= alloc((void**)&t->counters, 5u * 8u);
 	rc |= alloc((void**)&t->gstack, sfd_lbvh_walk_stack_bytes());
 	rc |= alloc((void**)&t->group_start, (size_t)sfd_lbvh_group_flag_len(N) * 4u);
@@ -188,12 +186,12 @@ int sf_lbvh_walk(sf_lbvh* t, unsigned int N, const float* pos, const float* mas,
 		return -1;
 	}
 	if (sfd_lbvh_records(N, t->node_lr, t->node_range, t->node_cm, t->node_cell, (float)p->theta, (int)p->leaf_size,
-	                     t->rec_b, t->node_q, quad, s))
+	                     quad, t->rec_b, s))
 		return -1;
 	/* G M l^p <= alpha |a| d^(p+2)  <=>  M l^p <= (alpha / G) |a| d^(p+2) */
 	const float alpha_g = (prev_acc && p->mac_alpha > 0.0) ? (float)(p->mac_alpha / p->G) : 0.0f;
 	return sfd_lbvh_walk(t->body, t->val0, N, t->group_start, t->node_cm, quad ? t->node_q0 : NULL,
-	                     quad ? t->node_q1 : NULL, t->rec_b, t->node_q, alpha_g, (float)(p->epsilon * p->epsilon),
+	                     quad ? t->node_q1 : NULL, t->rec_b, alpha_g, (float)(p->epsilon * p->epsilon),
 	                     p->G, p->acc_mode == SF_ACC_PHYSICAL, acc, pot, t->counters, t->gstack, s);
 }
 
