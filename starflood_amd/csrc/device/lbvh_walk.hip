@@ -106,8 +106,9 @@ __device__ __forceinline__ void eval_list(Acc& a, float xi, float yi, float zi, 
 //   pot += M / r + (1/2) (e^T Qt e) / r^5
 // (the potential -G [M/r + (1/2) d^T Qt d / r^5] with d = -e and its
 // gradient), Plummer-softened through r^2 = |e|^2 + eps^2.  Qt is
-// (q0 = Qxx, Qxy, Qxz, Qyy; q1 = Qyz), Qzz = -Qxx - Qyy.
-__device__ __forceinline__ void interact_quad(Acc& a, float xi, float yi, float zi, float4 c, float4 q0, float q1,
+// (q0 = Qxx, Qxy, Qxz, Qyy; q1 = Qyz, Qzz), Qzz = -Qxx - Qyy stored with the
+// queued cell.  34 VALU operations plus the rsqrt.
+__device__ __forceinline__ void interact_quad(Acc& a, float xi, float yi, float zi, float4 c, float4 q0, float2 q1,
                                               float eps2) {
 	const float ex = c.x - xi, ey = c.y - yi, ez = c.z - zi;
 	float r2 = __builtin_fmaf(ex, ex, eps2);
@@ -116,22 +117,22 @@ __device__ __forceinline__ void interact_quad(Acc& a, float xi, float yi, float 
 	const float inv = __builtin_amdgcn_rsqf(r2);
 	const float inv2 = inv * inv;
 	const float mi = c.w * inv;
-	const float inv3 = inv * inv2;
-	const float inv5 = inv3 * inv2;
-	const float qzz = -q0.x - q0.w;
+	const float m3 = mi * inv2;
+	const float inv5 = (inv * inv2) * inv2;
 	const float qx = __builtin_fmaf(q0.x, ex, __builtin_fmaf(q0.y, ey, q0.z * ez));
-	const float qy = __builtin_fmaf(q0.y, ex, __builtin_fmaf(q0.w, ey, q1 * ez));
-	const float qz = __builtin_fmaf(q0.z, ex, __builtin_fmaf(q1, ey, qzz * ez));
+	const float qy = __builtin_fmaf(q0.y, ex, __builtin_fmaf(q0.w, ey, q1.x * ez));
+	const float qz = __builtin_fmaf(q0.z, ex, __builtin_fmaf(q1.x, ey, q1.y * ez));
 	const float eqe = __builtin_fmaf(ex, qx, __builtin_fmaf(ey, qy, ez * qz));
-	const float w = __builtin_fmaf(2.5f * eqe * inv5, inv2, c.w * inv3);
+	const float t = eqe * inv5;
+	const float w = __builtin_fmaf(2.5f, t * inv2, m3);
 	a.tx = __builtin_fmaf(w, ex, __builtin_fmaf(-inv5, qx, a.tx));
 	a.ty = __builtin_fmaf(w, ey, __builtin_fmaf(-inv5, qy, a.ty));
 	a.tz = __builtin_fmaf(w, ez, __builtin_fmaf(-inv5, qz, a.tz));
-	a.tp += __builtin_fmaf(0.5f * inv5, eqe, mi);
+	a.tp = __builtin_fmaf(0.5f, t, a.tp) + mi;
 }
 
 __device__ __forceinline__ void eval_list_quad(Acc& a, float xi, float yi, float zi, const float4* c, const float4* q0,
-                                               const float* q1, int n, float eps2) {
+                                               const float2* q1, int n, float eps2) {
 	if (n == 64) {
 #pragma unroll 4
 		for (int k = 0; k < 64; ++k) interact_quad(a, xi, yi, zi, c[k], q0[k], q1[k], eps2);
@@ -159,10 +160,10 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
 	__shared__ int s_pid[kWaves][kList];
 	__shared__ float4 s_part[kWaves][64];
 	__shared__ float4 s_q0[kWaves][QUAD ? kList : 1];
-	__shared__ float s_q1[kWaves][QUAD ? kList : 1];
+	__shared__ float2 s_q1[kWaves][QUAD ? kList : 1];
 	const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
 	float4* cq0 = s_q0[w];
-	float* cq1 = s_q1[w];
+	float2* cq1 = s_q1[w];
 	int* stack = gstack + (size_t)(blockIdx.x * kWaves + w) * kStack;
 	float4* cells = s_cell[w];
 	int2* leaves = s_leaf[w];
@@ -269,8 +270,9 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
 						const int at = nc + __popcll(m_cell & lt);
 						cells[at] = ra;
 						if (QUAD) {
-							cq0[at] = node_q0[node];
-							cq1[at] = node_q1[node];
+							const float4 q0 = node_q0[node];
+							cq0[at] = q0;
+							cq1[at] = make_float2(node_q1[node], -q0.x - q0.w);
 						}
 					}
 					if (cls == 2) leaves[nl + __popcll(m_leaf & lt)] = make_int2(rb.y, rb.z);
