@@ -118,10 +118,11 @@ typedef struct {
 	                                default): a cell that passes the theta test is
 	                                also required to satisfy G M l^2 <= mac_alpha
 	                                |a|_min d^4 (quadrupole cells: G M l^3 <=
-	                                mac_alpha |a|_min d^5), with l its size, d its
+	                                mac_alpha |a|_min d^5, and those that also meet
+	                                G M l^2 <= 0.2 mac_alpha |a|_min d^4 are
+	                                summed as monopoles), with l its size, d its
 	                                distance to the walk group's box and |a|_min the
-	                                smallest
-	                                previous acceleration in the group (sim_step
+	                                smallest previous acceleration in the group (sim_step
 	                                reads it from the acc block; sim_solv and
 	                                solve_gravity_part_tree first walk with the theta
 	                                test alone).  Bounds the force error relative to

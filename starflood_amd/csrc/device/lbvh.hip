@@ -593,14 +593,13 @@ __global__ __launch_bounds__(kBlock) void k_node_moments(const float4* __restric
 // Walk records: (r_crit^2, a, b, q) with r_crit = side / theta +
 // |com - box centre| (node_cell's box).  Nodes of at most leaf_size particles
 // carry (a, b) = (first, count) and the sign bit of the r_crit^2 word set;
-// larger ones (left, right).  q is the size term of the relative opening
-// test: M l^2 for monopole cells, M l^3 for quadrupole cells (the leading
-// truncated term of each expansion), l = side, as float bits.
+// larger ones (left, right).  q is the node's size l = side (float bits),
+// which the relative opening test takes to M l^2 and M l^3.
 __global__ __launch_bounds__(kBlock) void k_records(int N, const int2* __restrict__ node_lr,
                                                     const int2* __restrict__ node_range,
                                                     const float4* __restrict__ node_cm,
                                                     const float4* __restrict__ node_cell, float inv_theta,
-                                                    int leaf_size, int cube, int4* __restrict__ rec_b) {
+                                                    int leaf_size, int4* __restrict__ rec_b) {
 	const int n = (int)(blockIdx.x * kBlock + threadIdx.x);
 	if (n >= 2 * N - 1) return;
 	int first, count, l = -1, r = -1;
@@ -618,9 +617,8 @@ __global__ __launch_bounds__(kBlock) void k_records(int N, const int2* __restric
 	const float ox = cm.x - cell.x, oy = cm.y - cell.y, oz = cm.z - cell.z;
 	const float rc = cell.w * inv_theta + sqrtf(ox * ox + oy * oy + oz * oz);
 	const bool leafy = count <= leaf_size;
-	const float ml2 = cm.w * cell.w * cell.w;
 	const int rc2 = __float_as_int(rc * rc) | (leafy ? (int)0x80000000u : 0);
-	rec_b[n] = make_int4(rc2, leafy ? first : l, leafy ? count : r, __float_as_int(cube ? ml2 * cell.w : ml2));
+	rec_b[n] = make_int4(rc2, leafy ? first : l, leafy ? count : r, __float_as_int(cell.w));
 }
 
 // Walk groups: the maximal subtrees holding at most `cap` particles.  They
@@ -785,13 +783,13 @@ int sfd_lbvh_quadrupoles(unsigned N, const void* body, const void* node_range, c
 }
 
 int sfd_lbvh_records(unsigned N, const void* node_lr, const void* node_range, const void* node_cm,
-                     const void* node_cell, float theta, int leaf_size, int quadrupole, void* rec_b, hipStream_t s) {
+                     const void* node_cell, float theta, int leaf_size, void* rec_b, hipStream_t s) {
 	const unsigned nn = 2u * N - 1u;
 	const float inv_theta = theta > 0.0f ? 1.0f / theta : 3.0e38f;
 	hipLaunchKernelGGL(k_records, dim3((nn + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (int)N,
 	                   static_cast<const int2*>(node_lr), static_cast<const int2*>(node_range),
 	                   static_cast<const float4*>(node_cm), static_cast<const float4*>(node_cell), inv_theta,
-	                   leaf_size, quadrupole ? 1 : 0, static_cast<int4*>(rec_b));
+	                   leaf_size, static_cast<int4*>(rec_b));
 	return sf_hip_check(hipGetLastError(), "sfd_lbvh_records", "launch");
 }
 

@@ -95,15 +95,16 @@ int sfd_lbvh_topology(const float* pos, const float* mas, const unsigned long lo
                       unsigned int N, const double* box, void* body, void* node_lr, void* node_range, void* prefix,
                       void* tile_sum, void* node_cm, void* node_cell, hipStream_t s);
 int sfd_lbvh_records(unsigned int N, const void* node_lr, const void* node_range, const void* node_cm,
-                     const void* node_cell, float theta, int leaf_size, int quadrupole, void* rec_b, hipStream_t s);
+                     const void* node_cell, float theta, int leaf_size, void* rec_b, hipStream_t s);
 /* lbvh_walk.hip.  counters (5 x u64; the first four zeroed here): cell
    interactions, particle interactions, wavefronts that met a malformed tree,
    node batches; the fifth adds the third up over walks until the host clears
    it.
    gstack: sfd_lbvh_walk_stack_bytes() of scratch.  alpha_g > 0 adds the
-   relative opening test M l^p <= alpha_g |a|_min d^(p+2) (the records' size
-   term M l^p, p = 2, or 3 for quadrupole cells), |a|_min over each group's
-   previous acc. */
+   relative opening test M l^p <= alpha_g |a|_min d^(p+2) (l the records'
+   size, p = 2 for monopole cells and 3 for quadrupole cells; a quadrupole
+   walk takes the cells that meet p = 2 as monopoles), |a|_min over each
+   group's previous acc. */
 size_t sfd_lbvh_walk_stack_bytes(void);
 int sfd_lbvh_walk(const void* body, const unsigned int* order, unsigned int N, const unsigned int* group_start,
                   const void* node_cm, const void* node_q0, const float* node_q1, const void* rec_b,

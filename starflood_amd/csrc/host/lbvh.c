@@ -186,7 +186,7 @@ int sf_lbvh_walk(sf_lbvh* t, unsigned int N, const float* pos, const float* mas,
 		return -1;
 	}
 	if (sfd_lbvh_records(N, t->node_lr, t->node_range, t->node_cm, t->node_cell, (float)p->theta, (int)p->leaf_size,
-	                     quad, t->rec_b, s))
+	                     t->rec_b, s))
 		return -1;
 	/* G M l^p <= alpha |a| d^(p+2)  <=>  M l^p <= (alpha / G) |a| d^(p+2) */
 	const float alpha_g = (prev_acc && p->mac_alpha > 0.0) ? (float)(p->mac_alpha / p->G) : 0.0f;

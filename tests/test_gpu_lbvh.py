@@ -273,6 +273,43 @@ def test_relative_opening_test_with_no_cells_equals_the_direct_sum(oracle):
     L.set_params()
 
 
+def test_quadrupole_walk_sums_cells_that_meet_the_monopole_bound_as_monopoles(oracle):
+    """Under the relative opening test a quadrupole walk sums a cell that
+    also meets G M l^2 <= 0.2 alpha |a| d^4 as a monopole, through the leaf
+    particles' queue.  With alpha 1e30 every accepted cell is such a
+    monopole: the forces equal the monopole walk's at the same theta (the
+    same cells, summed in another order) to FP32 rounding (median 1e-6, 99th
+    percentile 1e-5 of |a|), and the cells count as cell interactions.  At
+    alpha 2.5e-3 and theta 1.15 the 99th percentile stays within 1.5e-3,
+    far below the theta test's alone at theta 1.15 with either expansion."""
+    N = 131072
+    def run(quad, alpha, theta):
+        with lbvh_sim(N, theta=theta, mac_alpha=alpha, epsilon=5e-3, quadrupole=quad, leaf_size=4,
+                      mem_mode=L.SF_MEM_RESIDENT) as s:
+            s.ic_plummer(seed=9, first=0, count=N)
+            s.solve()
+            s.to_host()
+            ex = s.lbvh_export()
+            assert ex["bad"] == 0
+            idx = np.arange(0, N, 32, dtype=np.uint32)
+            a64, _ = s.direct_f64(idx)
+            return s.acc.copy(), idx, a64, ex
+    acc_m, idx, a64, ex_m = run(0, 0.0, 0.7)
+    acc_q, _, _, ex_q = run(1, 1e30, 0.7)
+    assert (ex_q["cells"], ex_q["parts"]) == (ex_m["cells"], ex_m["parts"])
+    d = np.linalg.norm(acc_q - acc_m, axis=1) / np.linalg.norm(acc_m, axis=1)
+    print(f"alpha 1e30 against the monopole walk: median {np.median(d):.3g}, p99 {np.percentile(d, 99):.3g}")
+    assert np.median(d) <= 1e-6 and np.percentile(d, 99) <= 1e-5
+    e_quad = rel_err(run(1, 0.0, 1.15)[0][idx], a64)
+    e_mix = rel_err(run(1, 2.5e-3, 1.15)[0][idx], a64)
+    e_mono = rel_err(run(0, 0.0, 1.15)[0][idx], a64)
+    print(f"p99 at theta 1.15: quadrupole {np.percentile(e_quad, 99):.3g}, relative alpha 2.5e-3 "
+          f"{np.percentile(e_mix, 99):.3g}, monopole {np.percentile(e_mono, 99):.3g}")
+    assert np.percentile(e_mix, 99) < min(np.percentile(e_mono, 99), np.percentile(e_quad, 99))
+    assert np.percentile(e_mix, 99) <= 1.5e-3
+    L.set_params()
+
+
 def test_mirror_and_resident_modes_give_identical_forces():
     N = 30000
     outs = []
