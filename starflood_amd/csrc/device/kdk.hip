@@ -70,14 +70,21 @@ __global__ __launch_bounds__(kBlock) void k_x64_refresh(const float* __restrict_
 	if ((float)x64[k] != p) x64[k] = (double)p;
 }
 
+// refresh: first apply k_x64_refresh's rule to x64 (a caller that always
+// writes pos from x64 passes 0)
 __global__ __launch_bounds__(kBlock) void k_kick_drift_fast(float* __restrict__ vel, const float* __restrict__ acc,
                                                             double* __restrict__ x64, float* __restrict__ pos,
-                                                            unsigned n3, float half_dt, double dt) {
+                                                            unsigned n3, float half_dt, double dt, int refresh) {
 	const unsigned k = blockIdx.x * kBlock + threadIdx.x;
 	if (k >= n3) return;
 	const float v = __builtin_fmaf(half_dt, acc[k], vel[k]);
 	vel[k] = v;
-	const double x = __builtin_fma(dt, (double)v, x64[k]);
+	double x0 = x64[k];
+	if (refresh) {
+		const float p = pos[k];
+		if ((float)x0 != p) x0 = (double)p;
+	}
+	const double x = __builtin_fma(dt, (double)v, x0);
 	x64[k] = x;
 	pos[k] = (float)x;
 }
@@ -86,6 +93,26 @@ __global__ __launch_bounds__(kBlock) void k_kick_fast(float* __restrict__ vel, c
                                                       unsigned n3, float half_dt) {
 	const unsigned k = blockIdx.x * kBlock + threadIdx.x;
 	if (k < n3) vel[k] = __builtin_fmaf(half_dt, acc[k], vel[k]);
+}
+
+// k_kick_fast followed by k_energy_terms, per particle, with the same
+// operations: one pass over vel instead of two launches.
+__global__ __launch_bounds__(kBlock) void k_kick_energy_fast(float* __restrict__ vel, const float* __restrict__ acc,
+                                                             const float* __restrict__ mas,
+                                                             const float* __restrict__ pot, unsigned N, float half_dt,
+                                                             float* __restrict__ ken, float* __restrict__ pen) {
+	const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= N) return;
+	const float vx = __builtin_fmaf(half_dt, acc[3u * i], vel[3u * i]);
+	const float vy = __builtin_fmaf(half_dt, acc[3u * i + 1u], vel[3u * i + 1u]);
+	const float vz = __builtin_fmaf(half_dt, acc[3u * i + 2u], vel[3u * i + 2u]);
+	vel[3u * i] = vx;
+	vel[3u * i + 1u] = vy;
+	vel[3u * i + 2u] = vz;
+	const float m = mas[i];
+	const float v2 = (vx * vx + vy * vy) + vz * vz;
+	ken[i] = 0.5f * m * v2;
+	pen[i] = m * pot[i];
 }
 
 constexpr int kRedQ = 11;   // T, V, W, U, Px, Py, Pz, Lx, Ly, Lz, K
@@ -202,13 +229,23 @@ extern "C" int sfd_x64_refresh(const float* pos, double* x64, unsigned n3, hipSt
 }
 
 extern "C" int sfd_kick_drift_fast(float* vel, const float* acc, double* x64, float* pos, unsigned n3,
-                                   float half_dt, double dt, hipStream_t s) {
+                                   float half_dt, double dt, int refresh, hipStream_t s) {
 	if (!n3) return 0;
 	const int tok = sf_prof_begin("kick_drift", s);
 	hipLaunchKernelGGL(k_kick_drift_fast, dim3(grid_for(n3)), dim3(kBlock), 0, s, vel, acc, x64, pos, n3, half_dt,
-	                   dt);
+	                   dt, refresh);
 	sf_prof_end(tok, s);
 	return sf_hip_check(hipGetLastError(), "sfd_kick_drift_fast", "launch");
+}
+
+extern "C" int sfd_kick_energy_fast(float* vel, const float* acc, const float* mas, const float* pot, unsigned N,
+                                    float half_dt, float* ken, float* pen, hipStream_t s) {
+	if (!N) return 0;
+	const int tok = sf_prof_begin("kick", s);
+	hipLaunchKernelGGL(k_kick_energy_fast, dim3(grid_for(N)), dim3(kBlock), 0, s, vel, acc, mas, pot, N, half_dt, ken,
+	                   pen);
+	sf_prof_end(tok, s);
+	return sf_hip_check(hipGetLastError(), "sfd_kick_energy_fast", "launch");
 }
 
 extern "C" int sfd_kick_fast(float* vel, const float* acc, unsigned n3, float half_dt, hipStream_t s) {

@@ -135,7 +135,7 @@ static hipStream_t stream_of(void* stream) { return (hipStream_t)stream; }
 static int kdk(sf_tape* t, hipStream_t s) {
 	const unsigned int n3 = 3u * t->N;
 	const float h = 0.5f * (float)t->p.dt;
-	if (sfd_kick_drift_fast(t->v, t->a, t->x64, t->x, n3, h, t->p.dt, s)) return -1;
+	if (sfd_kick_drift_fast(t->v, t->a, t->x64, t->x, n3, h, t->p.dt, 0, s)) return -1;
 	if (force(&t->p, &t->w, t->x, t->m, t->a, NULL, s)) return -1;
 	return sfd_kick_fast(t->v, t->a, n3, h, s);
 }

@@ -242,8 +242,8 @@ int sf_sim_step(sim_t* SF_RESTRICT sim_ptr) {
 		if (timing) tev_record(1, s);
 		if (sfd_drift_exact(pos, vel, n3, dt, s)) return STARFLOOD_FAILURE;       /* drift */
 	} else {
-		if (sfd_x64_refresh(pos, c->d_x64, n3, s)) return STARFLOOD_FAILURE;
-		if (sfd_kick_drift_fast(vel, acc, c->d_x64, pos, n3, half_dt, p->dt, s)) return STARFLOOD_FAILURE;
+		/* x64 restarts from pos where a host-side change left them apart */
+		if (sfd_kick_drift_fast(vel, acc, c->d_x64, pos, n3, half_dt, p->dt, 1, s)) return STARFLOOD_FAILURE;
 		if (timing) tev_record(1, s);
 	}
 	if (timing) tev_record(2, s);
@@ -258,11 +258,12 @@ int sf_sim_step(sim_t* SF_RESTRICT sim_ptr) {
 	if (exact) {
 		if (sfd_kick_exact(vel, acc, n3, half_dt, s)) return STARFLOOD_FAILURE;    /* kick_1 */
 	} else {
-		if (sfd_kick_fast(vel, acc, n3, half_dt, s)) return STARFLOOD_FAILURE;
+		/* kick_1 and the energy terms below, in one pass */
+		if (sfd_kick_energy_fast(vel, acc, mas, pot, N, half_dt, ken, pen, s)) return STARFLOOD_FAILURE;
 	}
 	if (cubic && sfd_sph_u_kick(c->d_u, c->d_dudt, N, half_dt, s)) return STARFLOOD_FAILURE;
 	if (timing) tev_record(4, s);
-	if (sfd_energy_terms(vel, mas, pot, N, ken, pen, s)) return STARFLOOD_FAILURE;
+	if (exact && sfd_energy_terms(vel, mas, pot, N, ken, pen, s)) return STARFLOOD_FAILURE;
 
 	double* red_out = c->d_red + (size_t)(sfd_reduce_scratch_doubles(N) - SF_RED_VALUES);
 	if (exact) {

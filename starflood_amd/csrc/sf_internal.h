@@ -127,11 +127,15 @@ int sfd_energy_terms(const float* vel, const float* mas, const float* pot, unsig
    written to *out.  One lane; for bit parity. */
 int sfd_energy_sum_sequential(const float* x, unsigned int N, double* out, hipStream_t s);
 
-/* FAST KDK: kick + FP64 drift fused; x64 refreshed from pos where they disagree. */
+/* FAST KDK: kick + FP64 drift fused; x64 refreshed from pos where they
+   disagree (sfd_x64_refresh, or inside the drift with refresh = 1). */
 int sfd_x64_refresh(const float* pos, double* x64, unsigned int n3, hipStream_t s);
 int sfd_kick_drift_fast(float* vel, const float* acc, double* x64, float* pos, unsigned int n3,
-                        float half_dt, double dt, hipStream_t s);
+                        float half_dt, double dt, int refresh, hipStream_t s);
 int sfd_kick_fast(float* vel, const float* acc, unsigned int n3, float half_dt, hipStream_t s);
+/* sfd_kick_fast then sfd_energy_terms in one pass over the particles */
+int sfd_kick_energy_fast(float* vel, const float* acc, const float* mas, const float* pot, unsigned int N,
+                         float half_dt, float* ken, float* pen, hipStream_t s);
 /* sph.hip: u += half_dt du/dt (cubic SPH model) */
 int sfd_sph_u_kick(float* u, const float* dudt, unsigned int N, float half_dt, hipStream_t s);
 
