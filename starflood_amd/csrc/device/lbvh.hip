@@ -229,51 +229,93 @@ __global__ __launch_bounds__(1024) void k_radix_scan(unsigned* __restrict__ hist
 	}
 }
 
-// Stable scatter: tile elements are ranked in index order; within a round of
-// 256 elements the rank among equal digits comes from a 64-lane match mask
-// built with eight ballots.
+// Stable scatter of one 4096-element tile.  Each wavefront ranks its 1024
+// consecutive elements in index order (a 64-lane match mask from eight
+// ballots per round, and per-wave running digit counts in LDS), the tile is
+// staged in LDS in sorted order, and it is written out by consecutive
+// threads, so that each digit's run goes to consecutive global addresses.
+// The tile's digit counts are differences of consecutive entries of the
+// scanned histogram (offs[256 * ntiles] would be n_total).
+constexpr unsigned kScatterPer = kTile / kBlock;          // elements per thread
+constexpr unsigned kWaveSpan = kTile / (kBlock / 64);     // elements per wavefront
+
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const unsigned long long* __restrict__ key_in,
                                                           const unsigned* __restrict__ val_in, unsigned shift,
                                                           unsigned ntiles, const unsigned* __restrict__ offs,
-                                                          unsigned long long* __restrict__ key_out,
+                                                          unsigned n_total, unsigned long long* __restrict__ key_out,
                                                           unsigned* __restrict__ val_out) {
-	__shared__ unsigned base_off[256];
-	__shared__ unsigned run[256];
-	__shared__ unsigned wcount[kBlock / 64][256];
+	__shared__ unsigned long long s_key[kTile];
+	__shared__ unsigned s_val[kTile];
+	__shared__ unsigned s_wcnt[kBlock / 64][256];   // running counts, then each wave's prefix
+	__shared__ unsigned s_start[256];               // the digit's first slot in the tile
+	__shared__ unsigned s_base[256];                // the digit's first global position of this tile
+	__shared__ unsigned s_wsum[kBlock / 64];
 	const unsigned t = threadIdx.x, lane = t & 63u, w = t >> 6;
-	base_off[t] = offs[t * ntiles + blockIdx.x];
-	run[t] = 0;
-	for (unsigned q = 0; q < kBlock / 64; ++q) wcount[q][t] = 0;
+	const unsigned at = t * ntiles + blockIdx.x;
+	const unsigned o = offs[at];
+	const unsigned cnt = (at + 1u < 256u * ntiles ? offs[at + 1u] : n_total) - o;
+	s_base[t] = o;
+	// exclusive scan of the tile's 256 digit counts
+	unsigned inc = cnt;
+	for (int k = 1; k < 64; k <<= 1) {
+		const unsigned v = __shfl_up(inc, (unsigned)k);
+		if ((int)lane >= k) inc += v;
+	}
+	if (lane == 63u) s_wsum[w] = inc;
+	for (unsigned q = 0; q < kBlock / 64; ++q) s_wcnt[q][t] = 0;
 	__syncthreads();
+	unsigned excl = inc - cnt;
+	for (unsigned q = 0; q < w; ++q) excl += s_wsum[q];
+	s_start[t] = excl;
+	// rank: element r of this thread is tile element w * kWaveSpan + 64 r + lane
 	const unsigned long long lt = (1ull << lane) - 1ull;
-	for (unsigned r = 0; r < kRadixRounds; ++r) {
-		const unsigned idx = blockIdx.x * kTile + r * kBlock + t;
-		const unsigned long long k = key_in[idx];
-		const unsigned v = val_in[idx];
+	const unsigned g0 = blockIdx.x * kTile + w * kWaveSpan + lane;
+	unsigned long long kr[kScatterPer];
+	unsigned vr[kScatterPer], pk[kScatterPer];
+#pragma unroll
+	for (unsigned r = 0; r < kScatterPer; ++r) {
+		const unsigned long long k = key_in[g0 + 64u * r];
 		const unsigned d = (unsigned)(k >> shift) & 255u;
 		unsigned long long match = ~0ull;
 		for (int bit = 0; bit < 8; ++bit) {
 			const unsigned long long b = __ballot((d >> bit) & 1u);
 			match &= ((d >> bit) & 1u) ? b : ~b;
 		}
+		const unsigned before = s_wcnt[w][d];
+		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 		const unsigned rank = __popcll(match & lt);
-		if (rank == 0) wcount[w][d] = __popcll(match);
-		__syncthreads();
-		unsigned before = run[d];
-		for (unsigned q = 0; q < w; ++q) before += wcount[q][d];
-		const unsigned dst = base_off[d] + before + rank;
-		key_out[dst] = k;
-		val_out[dst] = v;
-		__syncthreads();
-		unsigned add = 0;
-		for (unsigned q = 0; q < kBlock / 64; ++q) {
-			add += wcount[q][t];
-			wcount[q][t] = 0;
-		}
-		run[t] += add;
-		__syncthreads();
+		if (rank == 0u) s_wcnt[w][d] = before + (unsigned)__popcll(match);
+		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+		kr[r] = k;
+		vr[r] = val_in[g0 + 64u * r];
+		pk[r] = (d << 16) | (before + rank);   // before + rank < kWaveSpan
 	}
-	(void)ntiles;
+	__syncthreads();
+	// each wave's offset within every digit
+	unsigned run = 0;
+	for (unsigned q = 0; q < kBlock / 64; ++q) {
+		const unsigned c = s_wcnt[q][t];
+		s_wcnt[q][t] = run;
+		run += c;
+	}
+	__syncthreads();
+#pragma unroll
+	for (unsigned r = 0; r < kScatterPer; ++r) {
+		const unsigned d = pk[r] >> 16;
+		const unsigned slot = s_start[d] + s_wcnt[w][d] + (pk[r] & 0xFFFFu);
+		s_key[slot] = kr[r];
+		s_val[slot] = vr[r];
+	}
+	__syncthreads();
+#pragma unroll
+	for (unsigned r = 0; r < kScatterPer; ++r) {
+		const unsigned slot = r * kBlock + t;
+		const unsigned long long k = s_key[slot];
+		const unsigned d = (unsigned)(k >> shift) & 255u;
+		const unsigned dst = s_base[d] + slot - s_start[d];
+		key_out[dst] = k;
+		val_out[dst] = s_val[slot];
+	}
 }
 
 // ---------------------------------------------------------------- tree
@@ -712,7 +754,8 @@ int sfd_radix_sort_passes(unsigned long long* key0, unsigned* val0, unsigned lon
 			hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, chunk_sums, nchunks);
 			hipLaunchKernelGGL(k_scan_u32_down, dim3(nchunks), dim3(kBlock), 0, s, hist, nh, chunk_sums);
 		}
-		hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, shift, ntiles, hist, kout, vout);
+		hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, shift, ntiles, hist, n_sort,
+		                   kout, vout);
 	}
 	return sf_hip_check(hipGetLastError(), "sfd_radix_sort", "launch");
 }
