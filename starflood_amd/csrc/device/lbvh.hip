@@ -376,6 +376,10 @@ struct d4 {
 		return {m, m * (double)b.x, m * (double)b.y, m * (double)b.z};
 	}
 	__device__ d4 operator+(const d4& o) const { return {m + o.m, x + o.x, y + o.y, z + o.z}; }
+	__device__ d4 shfl_up(unsigned o) const {
+		return {__shfl_up(m, o), __shfl_up(x, o), __shfl_up(y, o), __shfl_up(z, o)};
+	}
+	__device__ d4 shfl(int src) const { return {__shfl(m, src), __shfl(x, src), __shfl(y, src), __shfl(z, src)}; }
 };
 struct d6 {   // xx, xy, xz, yy, yz, zz
 	double v[6];
@@ -389,10 +393,21 @@ struct d6 {   // xx, xy, xz, yy, yz, zz
 		for (int k = 0; k < 6; ++k) r.v[k] = v[k] + o.v[k];
 		return r;
 	}
+	__device__ d6 shfl_up(unsigned o) const {
+		d6 r;
+		for (int k = 0; k < 6; ++k) r.v[k] = __shfl_up(v[k], o);
+		return r;
+	}
+	__device__ d6 shfl(int src) const {
+		d6 r;
+		for (int k = 0; k < 6; ++k) r.v[k] = __shfl(v[k], src);
+		return r;
+	}
 };
 
 constexpr unsigned kScanPer = 16;
 constexpr unsigned kScanTile = kBlock * kScanPer;
+constexpr unsigned kScanWaveSpan = kScanTile / (kBlock / 64);   // consecutive elements per wavefront
 
 template <class R>
 __device__ R block_exclusive_scan(R v, R* sh) {
@@ -410,16 +425,40 @@ __device__ R block_exclusive_scan(R v, R* sh) {
 	return excl;
 }
 
+// Sum of a wavefront's kScanWaveSpan consecutive elements (read 64
+// consecutive at a time), reduced across lanes in a fixed order; every lane
+// returns it.
+template <class R>
+__device__ R wave_span_sum(const float4* __restrict__ body, unsigned N, const double* __restrict__ ref,
+                           unsigned first) {
+	const unsigned lane = threadIdx.x & 63u;
+	R s = R::zero();
+	for (unsigned r = 0; r < kScanPer; ++r) {
+		const unsigned i = first + 64u * r + lane;
+		if (i < N) s = s + R::of(body[i], ref);
+	}
+	for (unsigned o = 1; o < 64; o <<= 1) {
+		const R t = s.shfl_up(o);
+		if (lane >= o) s = s + t;
+	}
+	return s.shfl(63);
+}
+
+// Tile sums: each wavefront sums its kScanWaveSpan elements, then wave 0 adds
+// the four in order.
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_scan_tiles(const float4* __restrict__ body, unsigned N,
                                                        const double* __restrict__ ref, R* __restrict__ tile_sum) {
-	__shared__ R sh[kBlock];
-	const unsigned b0 = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
-	R s = R::zero();
-	for (unsigned k = 0; k < kScanPer; ++k)
-		if (b0 + k < N) s = s + R::of(body[b0 + k], ref);
-	const R excl = block_exclusive_scan(s, sh);
-	if (threadIdx.x == kBlock - 1) tile_sum[blockIdx.x] = excl + s;
+	__shared__ R sh[kBlock / 64];
+	const unsigned w = threadIdx.x >> 6;
+	const R s = wave_span_sum<R>(body, N, ref, blockIdx.x * kScanTile + w * kScanWaveSpan);
+	if ((threadIdx.x & 63u) == 0u) sh[w] = s;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		R t = sh[0];
+		for (unsigned q = 1; q < kBlock / 64; ++q) t = t + sh[q];
+		tile_sum[blockIdx.x] = t;
+	}
 }
 
 template <class R>
@@ -437,21 +476,31 @@ __global__ __launch_bounds__(kBlock) void k_scan_tile_sums(R* __restrict__ tile_
 	}
 }
 
+// Inclusive prefix of a tile: each wavefront starts from the tile offset plus
+// the sums of the wavefronts before it, and scans its elements 64 at a time
+// (a lane scan, then the round's total carried to the next round), so that
+// reads and writes are of consecutive records.
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_scan_write(const float4* __restrict__ body, unsigned N,
                                                        const double* __restrict__ ref, const R* __restrict__ tile_off,
                                                        R* __restrict__ prefix) {
-	__shared__ R sh[kBlock];
-	const unsigned b0 = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
-	R s = R::zero();
-	for (unsigned k = 0; k < kScanPer; ++k)
-		if (b0 + k < N) s = s + R::of(body[b0 + k], ref);
-	R run = tile_off[blockIdx.x] + block_exclusive_scan(s, sh);
-	for (unsigned k = 0; k < kScanPer; ++k) {
-		if (b0 + k < N) {
-			run = run + R::of(body[b0 + k], ref);
-			prefix[b0 + k] = run;   // inclusive
+	__shared__ R sh[kBlock / 64];
+	const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+	const unsigned first = blockIdx.x * kScanTile + w * kScanWaveSpan;
+	const R total = wave_span_sum<R>(body, N, ref, first);
+	if (lane == 0u) sh[w] = total;
+	__syncthreads();
+	R run = tile_off[blockIdx.x];
+	for (unsigned q = 0; q < w; ++q) run = run + sh[q];
+	for (unsigned r = 0; r < kScanPer; ++r) {
+		const unsigned i = first + 64u * r + lane;
+		R v = i < N ? R::of(body[i], ref) : R::zero();
+		for (unsigned o = 1; o < 64; o <<= 1) {
+			const R t = v.shfl_up(o);
+			if (lane >= o) v = v + t;
 		}
+		if (i < N) prefix[i] = run + v;   // inclusive
+		run = run + v.shfl(63);
 	}
 }
 
