@@ -480,6 +480,8 @@ def bench_bh_sweep(steps: int = 3, configs=None) -> dict:
         out["runs"].append({**cfg, "step_ms": step_ms, "walk_ms": w_ms / max(w_n, 1),
                             "interactions_per_particle": inter, "force_err_vs_fp64": err,
                             "force_err_after_steps": err_end})
+        print(f"sweep {len(out['runs'])}/{len(configs)}: {cfg} step {step_ms:.2f} ms, p99 {err['p99']:.2e}",
+              file=sys.stderr, flush=True)
     L.set_params()
     return out
 
