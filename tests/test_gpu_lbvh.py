@@ -310,6 +310,37 @@ def test_quadrupole_walk_sums_cells_that_meet_the_monopole_bound_as_monopoles(or
     L.set_params()
 
 
+@pytest.mark.parametrize("quad", [0, 1])
+@pytest.mark.parametrize("N", [1, 2, 3, 65, 1000])
+def test_relative_opening_test_on_small_and_degenerate_systems(oracle, N, quad):
+    """The relative opening test on tiny trees (solve, then a step, whose walk
+    takes the previous accelerations): no guard trips, and with theta 1e-12
+    every node opens, so the forces equal the FP64 direct sum.  A system of
+    coincident particles has |a| = 0 everywhere: the walk falls back to the
+    theta test and stays finite."""
+    rng = np.random.default_rng(100 + N)
+    with lbvh_sim(N, theta=1e-12, mac_alpha=2.5e-3, quadrupole=quad, leaf_size=4) as s:
+        s.pos[:] = rng.normal(size=(N, 3)).astype(np.float32)
+        s.mas[:] = rng.uniform(0.5, 1.5, N).astype(np.float32) / N
+        s.solve()
+        s.step_number = 0
+        s.step()
+        ex = s.lbvh_export()
+        assert ex["bad"] == 0 and ex["cells"] == 0 and ex["parts"] == N * N
+        a64, _ = oracle.pp_f64(s.pos.copy(), s.mas.copy(), 1e-2, acc_phys=1)
+        scale = np.linalg.norm(a64, axis=1).max() + 1e-30
+        assert np.abs(s.acc - a64).max() <= 2e-6 * scale
+    with lbvh_sim(N, theta=1.15, mac_alpha=2.5e-3, quadrupole=quad) as s:
+        s.pos[:] = 0.25
+        s.mas[:] = 1.0 / N
+        s.solve()
+        s.step_number = 0
+        s.step()
+        assert s.lbvh_export()["bad"] == 0
+        assert np.all(np.isfinite(s.acc)) and np.all(np.abs(s.acc) <= 1e-6)
+    L.set_params()
+
+
 def test_mirror_and_resident_modes_give_identical_forces():
     N = 30000
     outs = []
