@@ -15,6 +15,12 @@
 // interaction; leaf particles are gathered 64 at a time with one vector load,
 // so the tree logic and memory latency are amortised over 64 interactions.
 //
+// Options: quadrupole cells (the 34-VALU interact_quad), and the relative
+// opening test, which also requires G M l^p <= alpha |a|min d^(p+2) of a
+// cell (|a|min the group's smallest previous acceleration); a quadrupole
+// walk then sums the cells that meet the monopole form of that bound (scaled
+// by kMonoBeta) as monopoles, through the leaf particles' queue.
+//
 // The grid is persistent and sized to the wavefronts resident at once; each
 // wavefront takes every (grid size)-th run of 64 sorted positions, walks the
 // groups that start in it, and owns a node stack in global
