@@ -117,9 +117,12 @@ __device__ __forceinline__ unsigned long long spread21(unsigned long long v) {
 	return v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ pos, unsigned N, unsigned n_sort,
-                                                   const double* __restrict__ box, unsigned long long* __restrict__ key,
-                                                   unsigned* __restrict__ val) {
+// Keys, and (x, y, z, m) packed in index order, so that the gather after the
+// sort reads one 16-byte record per particle instead of two cache lines.
+__global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ pos, const float* __restrict__ mas,
+                                                   unsigned N, unsigned n_sort, const double* __restrict__ box,
+                                                   unsigned long long* __restrict__ key, unsigned* __restrict__ val,
+                                                   float4* __restrict__ packed) {
 	const unsigned i = blockIdx.x * kBlock + threadIdx.x;
 	if (i >= n_sort) return;
 	if (i >= N) {
@@ -128,14 +131,17 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ pos
 		return;
 	}
 	const double scale = box[3];
+	const float x = pos[3u * i], y = pos[3u * i + 1u], z = pos[3u * i + 2u];
+	const float c[3] = {x, y, z};
 	unsigned long long q[3];
 	for (int a = 0; a < 3; ++a) {
-		double t = ((double)pos[3u * i + a] - box[a]) * scale;
+		double t = ((double)c[a] - box[a]) * scale;
 		t = fmin(fmax(t, 0.0), 2097151.0);
 		q[a] = (unsigned long long)t;
 	}
 	key[i] = (spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
 	val[i] = i;
+	packed[i] = make_float4(x, y, z, mas[i]);
 }
 
 // ---------------------------------------------------------------- radix sort
@@ -319,13 +325,12 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const unsigned long lo
 }
 
 // ---------------------------------------------------------------- tree
-__global__ __launch_bounds__(kBlock) void k_gather_sorted(const float* __restrict__ pos, const float* __restrict__ mas,
+__global__ __launch_bounds__(kBlock) void k_gather_sorted(const float4* __restrict__ packed,
                                                           const unsigned* __restrict__ order, unsigned N,
                                                           float4* __restrict__ body) {
 	const unsigned p = blockIdx.x * kBlock + threadIdx.x;
 	if (p >= N) return;
-	const unsigned i = order[p];
-	body[p] = make_float4(pos[3u * i], pos[3u * i + 1u], pos[3u * i + 2u], mas[i]);
+	body[p] = packed[order[p]];
 }
 
 __device__ __forceinline__ int delta(const unsigned long long* __restrict__ key, int N, int i, int j) {
@@ -754,14 +759,14 @@ int sfd_lbvh_groups(unsigned N, const void* node_lr, const void* node_range, int
 
 unsigned sfd_lbvh_bbox_blocks(void) { return 256u; }
 
-int sfd_lbvh_keys(const float* pos, unsigned N, unsigned n_sort, double* bbox_part, double* box,
-                  unsigned long long* key, unsigned* val, hipStream_t s) {
+int sfd_lbvh_keys(const float* pos, const float* mas, unsigned N, unsigned n_sort, double* bbox_part, double* box,
+                  unsigned long long* key, unsigned* val, void* packed, hipStream_t s) {
 	const unsigned nb = sfd_lbvh_bbox_blocks();
 	if (N == 0 || n_sort % kTile != 0) return -1;
 	hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(kBlock), 0, s, pos, N, bbox_part);
 	hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, s, bbox_part, nb, box);
-	hipLaunchKernelGGL(k_morton, dim3((n_sort + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pos, N, n_sort, box, key,
-	                   val);
+	hipLaunchKernelGGL(k_morton, dim3((n_sort + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pos, mas, N, n_sort, box,
+	                   key, val, static_cast<float4*>(packed));
 	return sf_hip_check(hipGetLastError(), "sfd_lbvh_keys", "launch");
 }
 
@@ -828,9 +833,9 @@ int sfd_scan_u32_exclusive(unsigned* data, unsigned n, unsigned* sums, hipStream
 }
 
 // prefix: N records of 4 doubles; tile_sum: sfd_lbvh_scan_tiles(N) records
-int sfd_lbvh_topology(const float* pos, const float* mas, const unsigned long long* key, const unsigned* order,
-                      unsigned N, const double* box, void* body, void* node_lr, void* node_range, void* prefix,
-                      void* tile_sum, void* node_cm, void* node_cell, hipStream_t s) {
+int sfd_lbvh_topology(const unsigned long long* key, const unsigned* order, unsigned N, const double* box, void* body,
+                      void* node_lr, void* node_range, void* prefix, void* tile_sum, void* node_cm, void* node_cell,
+                      hipStream_t s) {
 	const unsigned g = (N + kBlock - 1) / kBlock;
 	float4* b4 = static_cast<float4*>(body);
 	int2* rg = static_cast<int2*>(node_range);
@@ -838,7 +843,9 @@ int sfd_lbvh_topology(const float* pos, const float* mas, const unsigned long lo
 	d4* ts = static_cast<d4*>(tile_sum);
 	const unsigned ntiles = sfd_lbvh_scan_tiles(N);
 	const int tok = sf_prof_begin("lbvh_topology", s);
-	hipLaunchKernelGGL(k_gather_sorted, dim3(g), dim3(kBlock), 0, s, pos, mas, order, N, b4);
+	// the packed records of sfd_lbvh_keys, in the prefix buffer until the scan overwrites it
+	hipLaunchKernelGGL(k_gather_sorted, dim3(g), dim3(kBlock), 0, s, reinterpret_cast<const float4*>(prefix), order, N,
+	                   b4);
 	if (N > 1)
 		hipLaunchKernelGGL(k_karras, dim3((N - 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, key, (int)N,
 		                   static_cast<int2*>(node_lr), rg);

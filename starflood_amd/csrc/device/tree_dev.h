@@ -72,9 +72,10 @@ unsigned int sfd_radix_tile(void);
 /* u32 entries of the radix-sort histogram scratch for n_sort keys */
 unsigned int sfd_radix_hist_entries(unsigned int n_sort);
 /* bbox (FP64), then 63-bit Morton keys and identity values for n_sort
-   (a multiple of the radix tile) entries; padding keys are all ones */
-int sfd_lbvh_keys(const float* pos, unsigned int N, unsigned int n_sort, double* bbox_part, double* box,
-                  unsigned long long* key, unsigned int* val, hipStream_t s);
+   (a multiple of the radix tile) entries; padding keys are all ones.
+   packed: N float4 (x, y, z, m) in index order, for sfd_lbvh_topology */
+int sfd_lbvh_keys(const float* pos, const float* mas, unsigned int N, unsigned int n_sort, double* bbox_part,
+                  double* box, unsigned long long* key, unsigned int* val, void* packed, hipStream_t s);
 /* stable 8-pass LSD radix sort; the result is in (key0, val0) */
 int sfd_radix_sort(unsigned long long* key0, unsigned int* val0, unsigned long long* key1, unsigned int* val1,
                    unsigned int n_sort, unsigned int* hist, hipStream_t s);
@@ -90,10 +91,11 @@ int sfd_scan_u32_exclusive(unsigned int* data, unsigned int n, unsigned int* sum
 /* tiles of the FP64 moment scan (records of 4 doubles each) */
 unsigned int sfd_lbvh_scan_tiles(unsigned int N);
 /* sorted bodies, radix tree, FP64 prefix moments (N records of 4 doubles),
-   node monopoles (2N-1 float4) and Morton cells (2N-1 float4: centre, side) */
-int sfd_lbvh_topology(const float* pos, const float* mas, const unsigned long long* key, const unsigned int* order,
-                      unsigned int N, const double* box, void* body, void* node_lr, void* node_range, void* prefix,
-                      void* tile_sum, void* node_cm, void* node_cell, hipStream_t s);
+   node monopoles (2N-1 float4) and Morton cells (2N-1 float4: centre, side).
+   prefix holds sfd_lbvh_keys' packed records on entry */
+int sfd_lbvh_topology(const unsigned long long* key, const unsigned int* order, unsigned int N, const double* box,
+                      void* body, void* node_lr, void* node_range, void* prefix, void* tile_sum, void* node_cm,
+                      void* node_cell, hipStream_t s);
 int sfd_lbvh_records(unsigned int N, const void* node_lr, const void* node_range, const void* node_cm,
                      const void* node_cell, float theta, int leaf_size, void* rec_b, hipStream_t s);
 /* lbvh_walk.hip.  counters (5 x u64; the first four zeroed here): cell

@@ -152,10 +152,11 @@ int sf_lbvh_build(sf_lbvh** st, unsigned int N, const float* pos, const float* m
 	t->n_sort = (N + tile - 1u) / tile * tile;
 	t->N = N;
 	const int tok = sf_prof_begin("lbvh_build", s);
-	if (sfd_lbvh_keys(pos, N, t->n_sort, t->bbox_part, t->box, t->key0, t->val0, s)) return -1;
+	/* the prefix buffer holds the packed (x, y, z, m) records until the topology's scan */
+	if (sfd_lbvh_keys(pos, mas, N, t->n_sort, t->bbox_part, t->box, t->key0, t->val0, t->prefix, s)) return -1;
 	if (sfd_radix_sort(t->key0, t->val0, t->key1, t->val1, t->n_sort, t->hist, s)) return -1;
-	if (sfd_lbvh_topology(pos, mas, t->key0, t->val0, N, t->box, t->body, t->node_lr, t->node_range, t->prefix,
-	                      t->tile_sum, t->node_cm, t->node_cell, s))
+	if (sfd_lbvh_topology(t->key0, t->val0, N, t->box, t->body, t->node_lr, t->node_range, t->prefix, t->tile_sum,
+	                      t->node_cm, t->node_cell, s))
 		return -1;
 	if (sfd_lbvh_groups(N, t->node_lr, t->node_range, SF_LBVH_GROUP, t->group_start, s)) return -1;
 	if (quadrupole) {
