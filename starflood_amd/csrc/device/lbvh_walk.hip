@@ -22,10 +22,10 @@
 // by kMonoBeta) as monopoles, through the leaf particles' queue.
 //
 // The grid is persistent and sized to the wavefronts resident at once; each
-// wavefront takes every (grid size)-th run of 64 sorted positions, walks the
-// groups that start in it, and owns a node stack in global
-// memory (deep enough that the 64-wide pops never have to narrow).  The self
-// pair is included in leaf sums, as in the reference's PP sum.
+// wavefront takes runs of 64 sorted positions from a shared counter, walks
+// the groups that start in each, and owns a node stack in global memory
+// (deep enough that the 64-wide pops never have to narrow).  The self pair
+// is included in leaf sums, as in the reference's PP sum.
 #include <hip/hip_runtime.h>
 
 #include "../sf_internal.h"
@@ -189,10 +189,14 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
 	unsigned long long n_cell_tot = 0, n_part_tot = 0, n_batch = 0;
 	unsigned bad = 0;
 
-	// static round-robin over chunks: a single shared work counter would make
-	// every wavefront wait on device-scope atomics that serialise across XCDs
-	const unsigned n_slots = gridDim.x * kWaves;
-	for (unsigned c = blockIdx.x * kWaves + w; c < n_chunks; c += n_slots) {
+	// chunks of 64 sorted positions are taken from a shared counter, one
+	// atomic per chunk: walk groups differ in cost (more so under the
+	// relative opening test), and a static split leaves waves idle at the end
+	for (;;) {
+		unsigned long long cw = 0;
+		if (lane == 0) cw = atomicAdd(&counters[5], 1ull);
+		const unsigned c = (unsigned)__shfl(cw, 0);
+		if (c >= n_chunks) break;
 		// the groups that start in sorted positions [64c, 64c + 64); each ends at
 		// the next start flag, at most 64 positions later (start[N] is set)
 		const unsigned at = 64u * c + lane;
@@ -424,7 +428,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
 }
 
 __global__ void k_walk_reset(unsigned long long* __restrict__ counters) {
-	if (threadIdx.x < 4) counters[threadIdx.x] = 0ull;
+	if (threadIdx.x < 6 && threadIdx.x != 4) counters[threadIdx.x] = 0ull;
 }
 
 // Blocks that are resident at once: every wavefront of the grid runs from the

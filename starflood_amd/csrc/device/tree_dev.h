@@ -98,10 +98,10 @@ int sfd_lbvh_topology(const unsigned long long* key, const unsigned int* order, 
                       void* node_cell, hipStream_t s);
 int sfd_lbvh_records(unsigned int N, const void* node_lr, const void* node_range, const void* node_cm,
                      const void* node_cell, float theta, int leaf_size, void* rec_b, hipStream_t s);
-/* lbvh_walk.hip.  counters (5 x u64; the first four zeroed here): cell
+/* lbvh_walk.hip.  counters (6 x u64; all but the fifth zeroed here): cell
    interactions, particle interactions, wavefronts that met a malformed tree,
    node batches; the fifth adds the third up over walks until the host clears
-   it.
+   it; the sixth hands out the chunks of 64 sorted positions.
    gstack: sfd_lbvh_walk_stack_bytes() of scratch.  alpha_g > 0 adds the
    relative opening test M l^p <= alpha_g |a|_min d^(p+2) (l the records'
    size, p = 2 for monopole cells and 3 for quadrupole cells; a quadrupole

@@ -31,7 +31,8 @@ struct sf_lbvh {
 	void* node_cm;            /* 2N-1 float4: com, mass */
 	void* node_cell;          /* 2N-1 float4: Morton cell centre, side */
 	void* rec_b;              /* 2N-1 int4 */
-	unsigned long long* counters;   /* 5: the walk's four (zeroed per walk), then the sticky guard count */
+	unsigned long long* counters;   /* 6: the walk's four (zeroed per walk), the sticky guard count, the
+	                                   walk's chunk counter (zeroed per walk) */
 	unsigned long long* h_counters; /* pinned */
 	int* gstack;                    /* walk node stacks, one per wavefront slot */
 	unsigned int* group_start;      /* walk group starts in sorted order */
@@ -95,12 +96,12 @@ static int ensure(sf_lbvh** st, unsigned int N) {
 	rc |= alloc(&t->node_cm, nn * 16u);
 	rc |= alloc(&t->node_cell, nn * 16u);
 	rc |= alloc(&t->rec_b, nn * 16u);
-	rc |= alloc((void**)&t->counters, 5u * 8u);
+	rc |= alloc((void**)&t->counters, 6u * 8u);
 	rc |= alloc((void**)&t->gstack, sfd_lbvh_walk_stack_bytes());
 	rc |= alloc((void**)&t->group_start, (size_t)sfd_lbvh_group_flag_len(N) * 4u);
 	if (!rc && hipHostMalloc((void**)&t->h_counters, 5u * 8u, hipHostMallocDefault) != hipSuccess) rc = -1;
 	/* the sticky guard count starts at zero (the walk resets only the first four) */
-	if (!rc && (hipMemset(t->counters, 0, 5u * 8u) != hipSuccess || hipDeviceSynchronize() != hipSuccess)) rc = -1;
+	if (!rc && (hipMemset(t->counters, 0, 6u * 8u) != hipSuccess || hipDeviceSynchronize() != hipSuccess)) rc = -1;
 	if (rc) {
 		sf_error("tree_build", "device allocation for an LBVH of %u particles failed", N);
 		free_buffers(t);
