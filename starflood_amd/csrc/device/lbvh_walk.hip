@@ -156,8 +156,8 @@ __device__ __forceinline__ void eval_list_quad(Acc& a, float xi, float yi, float
 }
 
 // 5 wavefronts per SIMD (96 VGPRs): the monopole walk fits without spills
-// and runs 7% faster at 4M than at 4 per SIMD; the quadrupole walk spills 20
-// VGPRs and is still 3.4% faster than at 4 per SIMD without spills.
+// and runs 7% faster at 4M than at 4 per SIMD; the quadrupole walk spills 11
+// VGPRs, and at 4 per SIMD without spills it was 3.4% slower.
 // QUAD: cells carry traceless quadrupoles (node_q0, node_q1).
 template <bool QUAD>
 __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ body, const unsigned* __restrict__ order,
@@ -431,8 +431,9 @@ __global__ void k_walk_reset(unsigned long long* __restrict__ counters) {
 	if (threadIdx.x < 6 && threadIdx.x != 4) counters[threadIdx.x] = 0ull;
 }
 
-// Blocks that are resident at once: every wavefront of the grid runs from the
-// start, so the static group split is balanced (cached after the first call).
+// Blocks that are resident at once, the size of the persistent grid: every
+// wavefront runs from the start and takes chunks until none are left
+// (cached after the first call).
 template <bool QUAD>
 unsigned resident_blocks_of(void) {
 	static unsigned cached = 0;
