@@ -38,6 +38,13 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kList = 128;          // queued cells / leaves / particle ids per wavefront
 constexpr int kStack = 8192;        // node ids per wavefront (global memory)
 constexpr int kHead = 160;          // room for a depth-first descent (tree depth <= 96)
+// The top of each wavefront's node stack lives in LDS: a batch's children
+// are pushed there and the next batch pops them without a round trip to L2.
+// Children that do not fit go to the global stack, which is popped when the
+// LDS top holds fewer than a batch.  The quadrupole walk's LDS (its cells
+// carry 24 more bytes) leaves room for a smaller top at 5 wavefronts per SIMD.
+constexpr int kTopMono = 256;
+constexpr int kTopQuad = 128;
 // A quadrupole walk under the relative opening test takes a cell as a
 // monopole when G M l^2 <= kMonoBeta alpha |a| d^4.  The factor calibrates
 // the two size terms against each other.  Swept at 4M particles (quadrupole
@@ -174,10 +181,13 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
 	__shared__ float4 s_part[kWaves][64];
 	__shared__ float4 s_q0[kWaves][QUAD ? kList : 1];
 	__shared__ float2 s_q1[kWaves][QUAD ? kList : 1];
+	constexpr int kTop = QUAD ? kTopQuad : kTopMono;
+	__shared__ int s_top[kWaves][kTop];
 	const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
 	float4* cq0 = s_q0[w];
 	float2* cq1 = s_q1[w];
 	int* stack = gstack + (size_t)(blockIdx.x * kWaves + w) * kStack;
+	int* top = s_top[w];
 	float4* cells = s_cell[w];
 	int2* leaves = s_leaf[w];
 	int* pid = s_pid[w];
@@ -240,16 +250,22 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
 			a.tx = a.ty = a.tz = a.tp = a.sx = a.sy = a.sz = a.sp = a.cx = a.cy = a.cz = a.cp = 0.0f;
 			unsigned long long n_cell = 0, n_part = 0, n_mono = 0;
 			unsigned popped = 0;
-			int sp = 1, nc = 0, nl = 0, npid = 0;
-			if (lane == 0) stack[0] = 0;   // root: internal node 0, or the only leaf when N == 1
+			// pending nodes: nt in the LDS top, ng in the global stack
+			int nt = 1, ng = 0, nc = 0, nl = 0, npid = 0;
+			if (lane == 0) top[0] = 0;   // root: internal node 0, or the only leaf when N == 1
 			wave_fence();
 
-			while (sp > 0 || nc > 0 || nl > 0 || npid > 0) {
-				if (sp > 0) {
-					const int k = (sp + 64 <= kStack - kHead) ? min(64, sp) : 1;
+			while (nt + ng > 0 || nc > 0 || nl > 0 || npid > 0) {
+				if (nt + ng > 0) {
+					// pops come from the LDS top first; the global stack grows by at
+					// most 2 k per batch, and by one per batch in the narrow regime
+					const int k = (ng + 128 <= kStack - kHead) ? min(64, nt + ng) : 1;
+					const int kt = min(k, nt);
 					int node = -1;
-					if ((int)lane < k) node = stack[sp - 1 - (int)lane];
-					sp -= k;
+					if ((int)lane < kt) node = top[nt - 1 - (int)lane];
+					else if ((int)lane < k) node = stack[ng - 1 - ((int)lane - kt)];
+					nt -= kt;
+					ng -= k - kt;
 					popped += (unsigned)k;
 					++n_batch;
 					wave_fence();
@@ -304,26 +320,35 @@ __global__ __launch_bounds__(kBlock, 5) void k_walk(const float4* __restrict__ b
 					}
 					if (cls == 2) leaves[nl + __popcll(m_leaf & lt)] = make_int2(rb.y, rb.z);
 					if (cls == 3) {
-						const int at = sp + 2 * __popcll(m_open & lt);
-						if (at + 1 < kStack) {   // always true: see kHead
-							stack[at] = rb.y;
-							stack[at + 1] = rb.z;
-						} else {
-							bad = 1u;
+						// children fill the LDS top; what does not fit goes to the
+						// global stack
+						const int at = nt + 2 * __popcll(m_open & lt);
+#pragma unroll
+						for (int h = 0; h < 2; ++h) {
+							const int child = h ? rb.z : rb.y;
+							if (at + h < kTop) {
+								top[at + h] = child;
+							} else {
+								const int gi = ng + at + h - kTop;
+								if (gi < kStack) stack[gi] = child;   // always: see kHead
+								else bad = 1u;
+							}
 						}
 					}
 					nc += __popcll(m_cell);
 					nl += __popcll(m_leaf);
 					npid += __popcll(m_mono);
 					n_mono += (unsigned long long)__popcll(m_mono);
-					sp = min(sp + 2 * __popcll(m_open), kStack);
+					const int pushed = nt + 2 * __popcll(m_open);
+					ng = min(ng + max(pushed - kTop, 0), kStack);
+					nt = min(pushed, kTop);
 					wave_fence();
 					if (popped > budget) {   // a tree visits each node at most once per group
 						bad = 1u;
-						sp = 0;
+						nt = ng = 0;
 					}
 				}
-				const bool drain = sp == 0;
+				const bool drain = nt + ng == 0;
 				// monopoles
 				if (nc >= 64 || (drain && nc > 0)) {
 					const int take = min(nc, 64);

@@ -98,16 +98,27 @@ def test_force_error_against_fp64_direct_sum(oracle, theta):
     L.set_params()
 
 
-def test_walk_is_deterministic():
-    N = 20000
+@pytest.mark.parametrize("quad,alpha", [(0, 0.0), (1, 0.0), (0, 1e-3), (1, 2.5e-3)])
+def test_walk_is_deterministic(quad, alpha):
+    """Wavefronts take their chunks of walk groups from a shared counter, so
+    which wavefront walks a group changes from run to run; a group's sums do
+    not depend on it, and the forces after sim_solv and two KDK steps are
+    bit-identical across runs, for monopole and quadrupole cells, with and
+    without the relative opening test."""
+    N = 200000
     outs = []
     for _ in range(2):
-        with lbvh_sim(N) as s:
+        with lbvh_sim(N, quadrupole=quad, mac_alpha=alpha, theta=1.15 if alpha else 0.5,
+                      leaf_size=6 if quad else 3) as s:
             s.ic_plummer(seed=9, first=0, count=N)
             s.solve()
-            outs.append((s.acc.copy(), s.pot.copy()))
-    assert np.array_equal(bits(outs[0][0]), bits(outs[1][0]))
-    assert np.array_equal(bits(outs[0][1]), bits(outs[1][1]))
+            for n in range(2):
+                s.step_number = n
+                s.step()
+            assert s.lbvh_export()["bad"] == 0
+            outs.append((s.acc.copy(), s.pot.copy(), s.pos.copy()))
+    for k in range(3):
+        assert np.array_equal(bits(outs[0][k]), bits(outs[1][k]))
     L.set_params()
 
 
