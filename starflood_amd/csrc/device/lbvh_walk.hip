@@ -23,8 +23,9 @@
 //
 // The grid is persistent and sized to the wavefronts resident at once; each
 // wavefront takes runs of 64 sorted positions from a shared counter, walks
-// the groups that start in each, and owns a node stack in global memory
-// (deep enough that the 64-wide pops never have to narrow).  The self pair
+// the groups that start in each, and owns a node stack: its top in LDS, the
+// rest in global memory (deep enough that the 64-wide pops rarely have to
+// narrow).  The self pair
 // is included in leaf sums, as in the reference's PP sum.
 #include <hip/hip_runtime.h>
 
