@@ -200,9 +200,12 @@ int sf_ic_plummer(sim_t* sim_ptr, uint32_t seed, unsigned int first, unsigned in
    batches the walk classified; node_cm = 2N-1 records
    (com x, y, z, mass), internal nodes first; node_lr_range = N-1 records
    (left, right, first, count), child ids >= N-1 being leaves of sorted
-   particle id - (N-1). */
+   particle id - (N-1); node_cell = 2N-1 records (centre x, y, z, side) of
+   the box in each node's opening test: its particles' bounding box for
+   nodes of at most 64 particles, its Morton cell above.  Any of them may be
+   NULL. */
 int sf_lbvh_export(const tree_t* tree_ptr, unsigned long long* keys, unsigned int* order,
-                   unsigned long long counts[4], float* node_cm, int* node_lr_range);
+                   unsigned long long counts[4], float* node_cm, int* node_lr_range, float* node_cell);
 
 /* --- kernel timing with HIP events on the library stream --- */
 void sf_prof_enable(int on);

@@ -119,7 +119,8 @@ _SIG = [
     ("sf_ic_uniform_ball", C.c_int, [C.POINTER(SimT), C.c_uint32, C.c_double, C.c_double, C.c_double]),
     ("sf_ic_plummer", C.c_int, [C.POINTER(SimT), C.c_uint32, C.c_uint, C.c_uint, C.c_double, C.c_double,
                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]),
-    ("sf_lbvh_export", C.c_int, [C.POINTER(TreeT), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("sf_lbvh_export", C.c_int, [C.POINTER(TreeT), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p]),
     ("sf_prof_enable", None, [C.c_int]),
     ("sf_prof_reset", None, []),
     ("sf_prof_query", C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_ulong)]),

@@ -666,24 +666,88 @@ __global__ __launch_bounds__(kBlock) void k_node_moments(const float4* __restric
 		cm = make_float4((float)centre[0], (float)centre[1], (float)centre[2], 0.0f);
 	}
 	node_cm[n] = cm;
-	if (rg.y <= kAabbNode) {
-		// the particles' own bounding box: its longest side and centre (exact
-		// FP32 min and max over the node's sorted range)
-		float lo[3] = {body[f].x, body[f].y, body[f].z}, hi[3] = {lo[0], lo[1], lo[2]};
-		for (int i = f + 1; i <= l; ++i) {
-			const float4 b = body[i];
-			lo[0] = fminf(lo[0], b.x);
-			lo[1] = fminf(lo[1], b.y);
-			lo[2] = fminf(lo[2], b.z);
-			hi[0] = fmaxf(hi[0], b.x);
-			hi[1] = fmaxf(hi[1], b.y);
-			hi[2] = fmaxf(hi[2], b.z);
-		}
-		const float side_box = fmaxf(fmaxf(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
-		node_cell[n] = make_float4(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2]), side_box);
-		return;
-	}
+	if (rg.y <= kAabbNode) return;   // its particles' box: k_group_boxes
 	node_cell[n] = make_float4((float)centre[0], (float)centre[1], (float)centre[2], (float)side);
+}
+
+// The opening-test boxes of the internal nodes of at most kAabbNode
+// particles: their particles' bounding box (centre, longest side).  These
+// are the internal nodes of the walk groups, the maximal subtrees of at most
+// 64 particles, and a radix-tree node's index is one end of its range, so a
+// group's internal nodes are the indices of its sorted range whose range
+// lies inside it.  One wavefront per group: a sparse table of min and max
+// over the group's particles (windows of 2^k particles, k = 0 .. 6) in LDS,
+// then one lane per internal node combines the two windows that cover its
+// range.  Min and max are exact, so each box equals a scan over the node's
+// range.
+constexpr int kBoxLevels = 7;
+constexpr unsigned kBoxWaves = kBlock / 64;
+static_assert(kAabbNode == 64, "the walk groups hold at most 64 particles");
+__global__ __launch_bounds__(kBlock) void k_group_boxes(const float4* __restrict__ body, int N,
+                                                        const unsigned* __restrict__ start,
+                                                        const int2* __restrict__ node_range,
+                                                        float4* __restrict__ node_cell) {
+	__shared__ float s_lo[kBoxWaves][kBoxLevels][3][64];
+	__shared__ float s_hi[kBoxWaves][kBoxLevels][3][64];
+	const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+	const unsigned c = blockIdx.x * kBoxWaves + w;   // chunk of 64 sorted positions
+	if (c >= ((unsigned)N + 63u) / 64u) return;
+	// the groups that start in the chunk; each ends at the next start flag,
+	// at most 64 positions later (start[N] is set)
+	const unsigned at = 64u * c + lane;
+	const unsigned long long e1 = __ballot(start[at] != 0u);
+	const unsigned long long e2 = __ballot(start[at + 64u] != 0u);
+	unsigned long long rem = e1 & __ballot(at < (unsigned)N);
+	while (rem) {
+		const int s0 = __builtin_ctzll(rem);
+		rem &= rem - 1ull;
+		const unsigned long long above = s0 < 63 ? e1 & (~0ull << (s0 + 1)) : 0ull;
+		const int e0 = above ? __builtin_ctzll(above) : (e2 ? 64 + __builtin_ctzll(e2) : 128);
+		const int base = 64 * (int)c + s0;
+		const int cnt = min(min(e0 - s0, 64), N - base);
+		if (cnt < 2) continue;   // a single particle: no internal node
+		const bool live = (int)lane < cnt;
+		const float4 b = body[live ? base + (int)lane : base];
+		float lo[3] = {b.x, b.y, b.z}, hi[3] = {b.x, b.y, b.z};
+#pragma unroll
+		for (int k = 0; k < kBoxLevels; ++k) {
+			if (k > 0) {
+				// window [j, j + 2^k) from [j, j + 2^(k-1)) and the next one; the
+				// windows that run past the group are never read
+				const unsigned d = 1u << (k - 1);
+#pragma unroll
+				for (int a = 0; a < 3; ++a) {
+					lo[a] = fminf(lo[a], __shfl_down(lo[a], d));
+					hi[a] = fmaxf(hi[a], __shfl_down(hi[a], d));
+				}
+			}
+#pragma unroll
+			for (int a = 0; a < 3; ++a) {
+				s_lo[w][k][a][lane] = lo[a];
+				s_hi[w][k][a][lane] = hi[a];
+			}
+		}
+		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+		const int idx = base + (int)lane;
+		if (live && idx < N - 1) {
+			const int2 rg = node_range[idx];
+			if (rg.x >= base && rg.x + rg.y <= base + cnt) {
+				const int a0 = rg.x - base, len = rg.y;
+				const int k = 31 - __clz(len);
+				const int a1 = a0 + len - (1 << k);
+				float l3[3], h3[3];
+#pragma unroll
+				for (int a = 0; a < 3; ++a) {
+					l3[a] = fminf(s_lo[w][k][a][a0], s_lo[w][k][a][a1]);
+					h3[a] = fmaxf(s_hi[w][k][a][a0], s_hi[w][k][a][a1]);
+				}
+				const float side_box = fmaxf(fmaxf(h3[0] - l3[0], h3[1] - l3[1]), h3[2] - l3[2]);
+				node_cell[idx] =
+				    make_float4(0.5f * (l3[0] + h3[0]), 0.5f * (l3[1] + h3[1]), 0.5f * (l3[2] + h3[2]), side_box);
+			}
+		}
+		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+	}
 }
 
 // Walk records: (r_crit^2, a, b, q) with r_crit = side / theta +
@@ -755,6 +819,16 @@ int sfd_lbvh_groups(unsigned N, const void* node_lr, const void* node_range, int
 	hipLaunchKernelGGL(k_group_flags, dim3((threads + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (int)N,
 	                   static_cast<const int2*>(node_lr), static_cast<const int2*>(node_range), cap, start);
 	return sf_hip_check(hipGetLastError(), "sfd_lbvh_groups", "launch");
+}
+
+int sfd_lbvh_group_boxes(unsigned N, const void* body, const unsigned* start, const void* node_range,
+                         void* node_cell, hipStream_t s) {
+	if (N < 2) return 0;
+	const unsigned chunks = (N + 63u) / 64u;
+	hipLaunchKernelGGL(k_group_boxes, dim3((chunks + kBoxWaves - 1) / kBoxWaves), dim3(kBlock), 0, s,
+	                   static_cast<const float4*>(body), (int)N, start, static_cast<const int2*>(node_range),
+	                   static_cast<float4*>(node_cell));
+	return sf_hip_check(hipGetLastError(), "sfd_lbvh_group_boxes", "launch");
 }
 
 unsigned sfd_lbvh_bbox_blocks(void) { return 256u; }

@@ -91,7 +91,8 @@ int sfd_scan_u32_exclusive(unsigned int* data, unsigned int n, unsigned int* sum
 /* tiles of the FP64 moment scan (records of 4 doubles each) */
 unsigned int sfd_lbvh_scan_tiles(unsigned int N);
 /* sorted bodies, radix tree, FP64 prefix moments (N records of 4 doubles),
-   node monopoles (2N-1 float4) and Morton cells (2N-1 float4: centre, side).
+   node monopoles (2N-1 float4) and Morton cells (2N-1 float4: centre, side;
+   internal nodes of at most 64 particles are left to sfd_lbvh_group_boxes).
    prefix holds sfd_lbvh_keys' packed records on entry */
 int sfd_lbvh_topology(const unsigned long long* key, const unsigned int* order, unsigned int N, const double* box,
                       void* body, void* node_lr, void* node_range, void* prefix, void* tile_sum, void* node_cm,
@@ -126,6 +127,11 @@ int sfd_lbvh_quadrupoles(unsigned int N, const void* body, const void* node_rang
 unsigned int sfd_lbvh_group_flag_len(unsigned int N);
 int sfd_lbvh_groups(unsigned int N, const void* node_lr, const void* node_range, int cap, unsigned int* start,
                     hipStream_t s);
+/* the particle boxes of the internal nodes of at most 64 particles into
+   node_cell (their other entries come from sfd_lbvh_topology); start: the
+   groups of sfd_lbvh_groups with cap 64 */
+int sfd_lbvh_group_boxes(unsigned int N, const void* body, const unsigned int* start, const void* node_range,
+                         void* node_cell, hipStream_t s);
 
 #ifdef __cplusplus
 }

@@ -160,6 +160,7 @@ int sf_lbvh_build(sf_lbvh** st, unsigned int N, const float* pos, const float* m
 	                      t->node_cm, t->node_cell, s))
 		return -1;
 	if (sfd_lbvh_groups(N, t->node_lr, t->node_range, SF_LBVH_GROUP, t->group_start, s)) return -1;
+	if (sfd_lbvh_group_boxes(N, t->body, t->group_start, t->node_range, t->node_cell, s)) return -1;
 	if (quadrupole) {
 		if (sfd_lbvh_quadrupoles(N, t->body, t->node_range, t->prefix, t->box, t->prefix2, t->tile_sum2, t->quad_ref,
 		                         t->node_q0, t->node_q1, s))
@@ -213,7 +214,7 @@ int sf_lbvh_guard(sf_lbvh* t, hipStream_t s, unsigned long long* tripped) {
 }
 
 int sf_lbvh_export_impl(sf_lbvh* t, hipStream_t s, unsigned long long* keys, unsigned int* order,
-                        unsigned long long counts[4], float* node_cm, int* node_lr_range) {
+                        unsigned long long counts[4], float* node_cm, int* node_lr_range, float* node_cell) {
 	if (!t || t->N == 0) {
 		sf_error("sf_lbvh_export", "no LBVH has been built");
 		return -1;
@@ -227,6 +228,8 @@ int sf_lbvh_export_impl(sf_lbvh* t, hipStream_t s, unsigned long long* keys, uns
 		for (int k = 0; k < 4; ++k) counts[k] = t->h_counters[k];
 	}
 	if (node_cm) SF_HIP(hipMemcpy(node_cm, t->node_cm, (2u * N - 1u) * 16u, hipMemcpyDeviceToHost), "sf_lbvh_export");
+	if (node_cell)
+		SF_HIP(hipMemcpy(node_cell, t->node_cell, (2u * N - 1u) * 16u, hipMemcpyDeviceToHost), "sf_lbvh_export");
 	if (node_lr_range && N > 1) {
 		int* tmp = (int*)malloc((N - 1u) * 8u);
 		if (!tmp) return -1;

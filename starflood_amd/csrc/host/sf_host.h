@@ -52,7 +52,7 @@ unsigned int sf_lbvh_size(const sf_lbvh* st);
 int sf_lbvh_guard(sf_lbvh* st, hipStream_t s, unsigned long long* tripped);
 /* synchronises s, then copies out what is asked for (NULL: skip) */
 int sf_lbvh_export_impl(sf_lbvh* st, hipStream_t s, unsigned long long* keys, unsigned int* order,
-                        unsigned long long counts[4], float* node_cm, int* node_lr_range);
+                        unsigned long long counts[4], float* node_cm, int* node_lr_range, float* node_cell);
 
 /* initcond.c: after a host-side write of sim.mem, refresh the device copy
    in SF_MEM_RESIDENT mode */

@@ -309,12 +309,13 @@ int sf_solve_gravity_tree(sf_ctx* c, tree_t* tree, unsigned int step_number, int
 }
 
 int sf_lbvh_export(const tree_t* tree_ptr, unsigned long long* keys, unsigned int* order,
-                   unsigned long long counts[4], float* node_cm, int* node_lr_range) {
+                   unsigned long long counts[4], float* node_cm, int* node_lr_range, float* node_cell) {
 	sf_tree_dev* d = tree_dev_lookup(tree_ptr, "sf_lbvh_export");
 	if (!d) return STARFLOOD_FAILURE;
 	SF_HIP(hipDeviceSynchronize(), "sf_lbvh_export");
-	return sf_lbvh_export_impl(d->lbvh, d->stream, keys, order, counts, node_cm, node_lr_range) ? STARFLOOD_FAILURE
-	                                                                                           : STARFLOOD_SUCCESS;
+	return sf_lbvh_export_impl(d->lbvh, d->stream, keys, order, counts, node_cm, node_lr_range, node_cell)
+	           ? STARFLOOD_FAILURE
+	           : STARFLOOD_SUCCESS;
 }
 
 /* After LBVH walks on stream s: fail if any wavefront of the walks since

@@ -185,13 +185,16 @@ class Simulation:
         counts = np.zeros(4, np.uint64)
         cm = np.zeros((2 * N - 1, 4), np.float32) if nodes else None
         lr = np.zeros((max(N - 1, 1), 4), np.int32) if nodes else None
+        cell = np.zeros((2 * N - 1, 4), np.float32) if nodes else None
         L.check(self.lib.sf_lbvh_export(C.byref(self.sim.tree), keys.ctypes.data, order.ctypes.data,
                                         counts.ctypes.data, cm.ctypes.data if nodes else None,
-                                        lr.ctypes.data if nodes else None), "sf_lbvh_export")
+                                        lr.ctypes.data if nodes else None, cell.ctypes.data if nodes else None),
+                "sf_lbvh_export")
         out = {"keys": keys, "order": order, "cells": int(counts[0]), "parts": int(counts[1]), "bad": int(counts[2]), "batches": int(counts[3])}
         if nodes:
             out["node_cm"] = cm
             out["node_lr_range"] = lr[:N - 1]
+            out["node_cell"] = cell
         return out
 
     def tree_nodes(self) -> np.ndarray:

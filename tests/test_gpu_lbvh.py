@@ -77,6 +77,56 @@ def test_morton_codes_sort_and_tree_invariants(oracle):
     L.set_params()
 
 
+def _check_node_boxes(ex, pos_sorted, N):
+    """Internal nodes of at most 64 particles: their opening-test box is the
+    particles' bounding box (centre, longest side), bit for bit as FP32 min,
+    max, 0.5 (lo + hi) and hi - lo; leaves: the particle, side 0; larger
+    nodes: a cell that holds their particles."""
+    cell, lr = ex["node_cell"], ex["node_lr_range"]
+    assert np.array_equal(bits(cell[N - 1:, :3]), bits(pos_sorted)) and not cell[N - 1:, 3].any()
+    small = 0
+    for n in range(N - 1):
+        f, c = int(lr[n, 2]), int(lr[n, 3])
+        p = pos_sorted[f:f + c]
+        lo, hi = p.min(0), p.max(0)
+        if c <= 64:
+            small += 1
+            want = np.append(np.float32(0.5) * (lo + hi), (hi - lo).max()).astype(np.float32)
+            assert np.array_equal(bits(cell[n]), bits(want)), (n, f, c, cell[n], want)
+        else:
+            half = 0.5 * np.float64(cell[n, 3]) * (1 + 1e-6) + 1e-6 * np.abs(cell[n, :3]).max()
+            assert np.all(np.abs(p - cell[n, :3].astype(np.float64)) <= half), n
+    assert small > 0
+
+
+def test_opening_test_boxes_of_small_nodes_are_their_particles_bounding_boxes(oracle):
+    """The boxes of nodes of at most 64 particles come from one wavefront per
+    walk group (a sparse table of min and max); they must equal a scan over
+    each node's sorted range, on a Plummer sphere with duplicated positions
+    and on the degenerate distributions."""
+    N = 20000
+    with lbvh_sim(N) as s:
+        s.ic_plummer(seed=4, first=0, count=N)
+        s.pos[:5] = s.pos[5:10]
+        s.solve()
+        ex = s.lbvh_export(nodes=True)
+        _check_node_boxes(ex, s.pos[ex["order"]], N)
+    rng = np.random.default_rng(5)
+    for name, pos in degenerate_cases(3000, rng):
+        with lbvh_sim(3000) as s:
+            s.pos[:] = pos.astype(np.float32)
+            s.solve()
+            ex = s.lbvh_export(nodes=True)
+            _check_node_boxes(ex, s.pos[ex["order"]], 3000)
+    for n in (2, 3, 64, 65, 129):
+        with lbvh_sim(n) as s:
+            s.ic_plummer(seed=n, first=0, count=n)
+            s.solve()
+            ex = s.lbvh_export(nodes=True)
+            _check_node_boxes(ex, s.pos[ex["order"]], n)
+    L.set_params()
+
+
 @pytest.mark.parametrize("theta", [0.3, 0.5, 0.7])
 def test_force_error_against_fp64_direct_sum(oracle, theta):
     N = 131072

@@ -35,7 +35,7 @@ def test_adjoint_entry_points_validate_arguments():
 def test_lbvh_export_without_a_tree_fails_cleanly():
     lib = L.load()
     t = L.TreeT()
-    assert lib.sf_lbvh_export(C.byref(t), None, None, None, None, None) == -1
+    assert lib.sf_lbvh_export(C.byref(t), None, None, None, None, None, None) == -1
 
 
 def test_sf_run_rejects_bad_configurations_before_any_device_work(tmp_path):
