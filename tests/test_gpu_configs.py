@@ -182,6 +182,25 @@ def test_config5_bh_4m_theta_sweep():
     L.set_params()
 
 
+def test_lbvh_at_16m_particles():
+    """Four times the largest configured workload: Plummer N = 2^24 (eps =
+    0.002), monopole cells at theta 0.5 and quadrupole cells under the
+    relative opening test (theta 1.15, mac_alpha 2.5e-3, leaf 6), sim_solv
+    and 2 KDK steps each.  The trees are deeper and every index 4x larger;
+    no walk may be cut short (sim_solv, the statistics and the copy back
+    fail if one is), and the errors on every 4096th particle stay within the
+    bounds of the 4M runs (measured p99 2.9e-3 and 8.6e-4, energy drift
+    3.3e-7 and 6e-8)."""
+    N = 1 << 24
+    print(f"BH N={N} theta=0.5 2 steps", flush=True)
+    err, drift = bh_run(N, 2, 2e-3, 7, 4096, theta=0.5)
+    assert err[50] <= 1e-3 and err[99] <= 5e-3 and drift <= 1e-5
+    print(f"BH N={N} quadrupole theta=1.15 mac_alpha=2.5e-3 2 steps", flush=True)
+    err, drift = bh_run(N, 2, 2e-3, 7, 4096, theta=1.15, leaf_size=6, quadrupole=1, mac_alpha=2.5e-3)
+    assert err[99] <= 1e-3 and drift <= 1e-5
+    L.set_params()
+
+
 def test_config5_pp_262144_10_steps():
     """PP gravity N = 262144 (Plummer, eps = 0.002), 10 KDK steps in the fast
     policy: every acceleration of every 64th particle within 1e-5 of the
