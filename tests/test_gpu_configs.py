@@ -13,6 +13,8 @@ through the C ABI, with its accuracy stated against FP64:
             (quadrupole cells at 0.55; the relative opening test with
             quadrupole and monopole cells), eps 0.002, 10 steps: error
             percentiles and energy drift
+  beyond     the Barnes-Hut path at N = 2^24, four times the largest
+            configured workload
 
 The FP64 direct sum on the device (sf_pp_direct_f64) is pinned to the CPU
 oracle by test_gpu_pp.py; the bounds below are stated per test."""
