@@ -687,8 +687,8 @@ __global__ __launch_bounds__(kBlock) void k_group_boxes(const float4* __restrict
                                                         const unsigned* __restrict__ start,
                                                         const int2* __restrict__ node_range,
                                                         float4* __restrict__ node_cell) {
-	__shared__ float s_lo[kBoxWaves][kBoxLevels][3][64];
-	__shared__ float s_hi[kBoxWaves][kBoxLevels][3][64];
+	// one table at a time (the minima, then the maxima): 7 KiB per wavefront
+	__shared__ float s_tab[kBoxWaves][kBoxLevels][3][64];
 	const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
 	const unsigned c = blockIdx.x * kBoxWaves + w;   // chunk of 64 sorted positions
 	if (c >= ((unsigned)N + 63u) / 64u) return;
@@ -698,6 +698,17 @@ __global__ __launch_bounds__(kBlock) void k_group_boxes(const float4* __restrict
 	const unsigned long long e1 = __ballot(start[at] != 0u);
 	const unsigned long long e2 = __ballot(start[at + 64u] != 0u);
 	unsigned long long rem = e1 & __ballot(at < (unsigned)N);
+	if (!rem) return;
+	// the bodies and node ranges of the chunk and of the part of the next one
+	// that its last group reaches, loaded once and handed to each group by
+	// lane shuffles
+	const int over = e2 ? __builtin_ctzll(e2) : 64;
+	const int p0 = (int)at, p1 = p0 + 64;
+	const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+	const float4 b0 = p0 < N ? body[p0] : zero4;
+	const float4 b1 = ((int)lane < over && p1 < N) ? body[p1] : zero4;
+	const int2 r0 = p0 < N - 1 ? node_range[p0] : make_int2(0, 0);
+	const int2 r1 = ((int)lane < over && p1 < N - 1) ? node_range[p1] : make_int2(0, 0);
 	while (rem) {
 		const int s0 = __builtin_ctzll(rem);
 		rem &= rem - 1ull;
@@ -707,46 +718,59 @@ __global__ __launch_bounds__(kBlock) void k_group_boxes(const float4* __restrict
 		const int cnt = min(min(e0 - s0, 64), N - base);
 		if (cnt < 2) continue;   // a single particle: no internal node
 		const bool live = (int)lane < cnt;
-		const float4 b = body[live ? base + (int)lane : base];
-		float lo[3] = {b.x, b.y, b.z}, hi[3] = {b.x, b.y, b.z};
-#pragma unroll
-		for (int k = 0; k < kBoxLevels; ++k) {
-			if (k > 0) {
-				// window [j, j + 2^k) from [j, j + 2^(k-1)) and the next one; the
-				// windows that run past the group are never read
-				const unsigned d = 1u << (k - 1);
-#pragma unroll
-				for (int a = 0; a < 3; ++a) {
-					lo[a] = fminf(lo[a], __shfl_down(lo[a], d));
-					hi[a] = fmaxf(hi[a], __shfl_down(hi[a], d));
-				}
-			}
-#pragma unroll
-			for (int a = 0; a < 3; ++a) {
-				s_lo[w][k][a][lane] = lo[a];
-				s_hi[w][k][a][lane] = hi[a];
-			}
-		}
-		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+		// position base + lane: lane s0 + lane of the chunk's loads, or of the
+		// next chunk's.  Every lane takes part in every shuffle (a lane left
+		// out of a shuffle does not provide its value to the others), and
+		// then picks its half.
+		const int src = (s0 + (int)lane) & 63;
+		const bool hi_half = s0 + (int)lane >= 64;
+		const float x0 = __shfl(b0.x, src), y0 = __shfl(b0.y, src), z0 = __shfl(b0.z, src);
+		const float x1 = __shfl(b1.x, src), y1 = __shfl(b1.y, src), z1 = __shfl(b1.z, src);
+		const int f0 = __shfl(r0.x, src), n0 = __shfl(r0.y, src);
+		const int f1 = __shfl(r1.x, src), n1 = __shfl(r1.y, src);
+		const float4 b = hi_half ? make_float4(x1, y1, z1, 0.f) : make_float4(x0, y0, z0, 0.f);
+		const int2 rg = hi_half ? make_int2(f1, n1) : make_int2(f0, n0);
 		const int idx = base + (int)lane;
-		if (live && idx < N - 1) {
-			const int2 rg = node_range[idx];
-			if (rg.x >= base && rg.x + rg.y <= base + cnt) {
-				const int a0 = rg.x - base, len = rg.y;
-				const int k = 31 - __clz(len);
-				const int a1 = a0 + len - (1 << k);
-				float l3[3], h3[3];
+		// an internal node of this group: its range lies inside the group
+		const bool node = live && idx < N - 1 && rg.x >= base && rg.x + rg.y <= base + cnt;
+		const int a0 = rg.x - base, len = max(rg.y, 1);
+		const int k = 31 - __clz(len);
+		const int a1 = a0 + len - (1 << k);
+		float l3[3], h3[3];
+#pragma unroll
+		for (int pass = 0; pass < 2; ++pass) {
+			float v[3] = {b.x, b.y, b.z};
+#pragma unroll
+			for (int lv = 0; lv < kBoxLevels; ++lv) {
+				if (lv > 0) {
+					// window [j, j + 2^lv) from [j, j + 2^(lv-1)) and the next one;
+					// the windows that run past the group are never read
+					const unsigned d = 1u << (lv - 1);
+#pragma unroll
+					for (int a = 0; a < 3; ++a) {
+						const float o = __shfl_down(v[a], d);
+						v[a] = pass ? fmaxf(v[a], o) : fminf(v[a], o);
+					}
+				}
+#pragma unroll
+				for (int a = 0; a < 3; ++a) s_tab[w][lv][a][lane] = v[a];
+			}
+			__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+			if (node) {
 #pragma unroll
 				for (int a = 0; a < 3; ++a) {
-					l3[a] = fminf(s_lo[w][k][a][a0], s_lo[w][k][a][a1]);
-					h3[a] = fmaxf(s_hi[w][k][a][a0], s_hi[w][k][a][a1]);
+					const float x0 = s_tab[w][k][a][a0], x1 = s_tab[w][k][a][a1];
+					if (pass) h3[a] = fmaxf(x0, x1);
+					else l3[a] = fminf(x0, x1);
 				}
-				const float side_box = fmaxf(fmaxf(h3[0] - l3[0], h3[1] - l3[1]), h3[2] - l3[2]);
-				node_cell[idx] =
-				    make_float4(0.5f * (l3[0] + h3[0]), 0.5f * (l3[1] + h3[1]), 0.5f * (l3[2] + h3[2]), side_box);
 			}
+			__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 		}
-		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+		if (node) {
+			const float side_box = fmaxf(fmaxf(h3[0] - l3[0], h3[1] - l3[1]), h3[2] - l3[2]);
+			node_cell[idx] =
+			    make_float4(0.5f * (l3[0] + h3[0]), 0.5f * (l3[1] + h3[1]), 0.5f * (l3[2] + h3[2]), side_box);
+		}
 	}
 }
 
