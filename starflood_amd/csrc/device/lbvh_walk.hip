@@ -25,8 +25,8 @@
 // wavefront takes runs of 64 sorted positions from a shared counter, walks
 // the groups that start in each, and owns a node stack: its top in LDS, the
 // rest in global memory (deep enough that the 64-wide pops rarely have to
-// narrow).  The self pair
-// is included in leaf sums, as in the reference's PP sum.
+// narrow).  The self pair is included in leaf sums, as in the reference's PP
+// sum.
 #include <hip/hip_runtime.h>
 
 #include "../sf_internal.h"
